@@ -1,0 +1,542 @@
+// api.hip -- the C ABI of libmodp_hip.so: context, memory, argument checks
+// (mirroring the reference's exceptions as status codes) and dispatch.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "common.h"
+#include "batched.h"
+
+#define MODP_VERSION "modp_hip 0.1.0 (gfx950)"
+
+// ---------------------------------------------------------------------------
+// errors, workspaces, timing
+// ---------------------------------------------------------------------------
+
+int set_error(modp_ctx *ctx, int code, const char *fmt, ...)
+{
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->last_error = buf;
+    }
+    return code;
+}
+
+int hip_check(modp_ctx *ctx, hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return MODP_OK;
+    if (e == hipErrorOutOfMemory) return set_error(ctx, MODP_ERR_OOM, "%s: out of device memory", what);
+    return set_error(ctx, MODP_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+void *ws_get(modp_ctx *ctx, int slot, size_t bytes, int *status)
+{
+    if (bytes == 0) bytes = 256;
+    if (ctx->ws_size[slot] >= bytes) return ctx->ws[slot];
+    if (ctx->ws[slot]) {
+        // in-flight kernels may still read the old buffer
+        hipStreamSynchronize(ctx->stream);
+        hipFree(ctx->ws[slot]);
+        ctx->ws[slot] = nullptr;
+        ctx->ws_size[slot] = 0;
+    }
+    size_t sz = (bytes + 4095) & ~(size_t)4095;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess) {
+        *status = hip_check(ctx, e, "workspace allocation");
+        return nullptr;
+    }
+    ctx->ws[slot] = p;
+    ctx->ws_size[slot] = sz;
+    return p;
+}
+
+void *pinned_get(modp_ctx *ctx, size_t bytes, int *status)
+{
+    if (ctx->pinned_size >= bytes) return ctx->pinned;
+    if (ctx->pinned) {
+        hipStreamSynchronize(ctx->stream);
+        hipHostFree(ctx->pinned);
+        ctx->pinned = nullptr;
+        ctx->pinned_size = 0;
+    }
+    size_t sz = bytes < 4096 ? 4096 : bytes;
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, sz, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *status = hip_check(ctx, e, "pinned allocation");
+        return nullptr;
+    }
+    ctx->pinned = p;
+    ctx->pinned_size = sz;
+    return p;
+}
+
+static hipEvent_t take_event(modp_ctx *ctx)
+{
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+TimedLaunch::TimedLaunch(modp_ctx *c, const char *name) : ctx(c), t(nullptr)
+{
+    if (!ctx->timing) return;
+    ctx->timings.push_back(KernelTiming{name, take_event(ctx), take_event(ctx)});
+    t = &ctx->timings.back();
+    hipEventRecord(t->a, ctx->stream);
+}
+
+TimedLaunch::~TimedLaunch()
+{
+    if (t) hipEventRecord(ctx->timings.back().b, ctx->stream);
+}
+
+// ---------------------------------------------------------------------------
+// argument checking helpers
+// ---------------------------------------------------------------------------
+
+static int kind_of(modp_ctx *ctx, int dtype, int *kind)
+{
+    if (dtype == MODP_F64) { *kind = EK_F64; return MODP_OK; }
+    if (dtype == MODP_F32) { *kind = EK_F32; return MODP_OK; }
+    return set_error(ctx, MODP_ERR_DTYPE, "unsupported element type %d", dtype);
+}
+
+static int check_mat(modp_ctx *ctx, const char *name, const void *X, int64_t rows, int64_t cols, int64_t ld)
+{
+    if (rows < 0 || cols < 0) return set_error(ctx, MODP_ERR_DIM, "%s: negative dimension", name);
+    if (rows > 0 && cols > 0 && !X) return set_error(ctx, MODP_ERR_ARG, "%s: null pointer", name);
+    if (ld < (rows > 1 ? rows : 1)) return set_error(ctx, MODP_ERR_ARG, "%s: leading dimension %lld < rows %lld", name, (long long)ld, (long long)rows);
+    return MODP_OK;
+}
+
+// largest stripe that keeps partial sums exact in the storage type
+// (find_max_stripe_ops, stripe_mul.jl:13-27): floor((2^bits-1)/(N-1)^2) - 1
+static int check_modulus_gemm(modp_ctx *ctx, int dtype, int64_t p)
+{
+    if (p < 2) return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld < 2", (long long)p);
+    const int bits = dtype == MODP_F64 ? 53 : 24;
+    const long double top = ldexpl(1.0L, bits) - 1.0L;
+    const long double d = (long double)(p - 1) * (long double)(p - 1);
+    const long double M = floorl(top / d) - 1.0L;
+    if (M < 1.0L)
+        return set_error(ctx, MODP_ERR_MODULUS_RANGE,
+                         "cannot perform a single multiplication for modulus %lld with datatype %s", (long long)p,
+                         dtype == MODP_F64 ? "Float64" : "Float32");
+    return MODP_OK;
+}
+
+static int check_modulus_elim(modp_ctx *ctx, int dtype, int64_t p)
+{
+    if (p < 2) return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld < 2", (long long)p);
+    // residues and pivot-row products must be exact: p <= 2^26 (Float64);
+    // Float32 storage holds residues exactly up to 2^24
+    const int64_t lim = dtype == MODP_F64 ? ((int64_t)1 << 26) : ((int64_t)1 << 24);
+    if (p > lim)
+        return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld too large for elimination with %s storage",
+                         (long long)p, dtype == MODP_F64 ? "Float64" : "Float32");
+    return MODP_OK;
+}
+
+#define CTX_CHECK(ctx) \
+    if (!(ctx)) return MODP_ERR_ARG; \
+    (ctx)->last_error.clear();
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char *modp_version(void) { return MODP_VERSION; }
+
+int modp_device_count(int *count)
+{
+    if (!count) return MODP_ERR_ARG;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) { *count = 0; return MODP_ERR_NO_DEVICE; }
+    *count = n;
+    return MODP_OK;
+}
+
+int modp_ctx_create(int device, modp_ctx **out)
+{
+    if (!out) return MODP_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MODP_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return MODP_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return MODP_ERR_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return MODP_ERR_DEVICE;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return MODP_ERR_NO_DEVICE;
+    modp_ctx *c = new modp_ctx();
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return MODP_ERR_DEVICE;
+    }
+    *out = c;
+    return MODP_OK;
+}
+
+int modp_ctx_destroy(modp_ctx *ctx)
+{
+    if (!ctx) return MODP_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 8; ++i)
+        if (ctx->ws[i]) hipFree(ctx->ws[i]);
+    if (ctx->pinned) hipHostFree(ctx->pinned);
+    for (auto &t : ctx->timings) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+    for (auto e : ctx->event_pool) hipEventDestroy(e);
+    for (auto e : ctx->slots)
+        if (e) hipEventDestroy(e);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MODP_OK;
+}
+
+const char *modp_last_error(modp_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int modp_synchronize(modp_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    return hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+void *modp_stream(modp_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int modp_malloc(modp_ctx *ctx, size_t bytes, void **dptr)
+{
+    CTX_CHECK(ctx);
+    if (!dptr) return set_error(ctx, MODP_ERR_ARG, "null output pointer");
+    *dptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    return hip_check(ctx, hipMalloc(dptr, bytes), "hipMalloc");
+}
+
+int modp_free(modp_ctx *ctx, void *dptr)
+{
+    CTX_CHECK(ctx);
+    if (!dptr) return MODP_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return hip_check(ctx, hipFree(dptr), "hipFree");
+}
+
+int modp_memcpy_h2d(modp_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    CTX_CHECK(ctx);
+    if (bytes == 0) return MODP_OK;
+    HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipMemcpy H2D");
+}
+
+int modp_memcpy_d2h(modp_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    CTX_CHECK(ctx);
+    if (bytes == 0) return MODP_OK;
+    HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    return hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipMemcpy D2H");
+}
+
+int modp_memcpy_d2d(modp_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    CTX_CHECK(ctx);
+    if (bytes == 0) return MODP_OK;
+    return hip_check(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream), "hipMemcpy D2D");
+}
+
+int modp_memset_zero(modp_ctx *ctx, void *dptr, size_t bytes)
+{
+    CTX_CHECK(ctx);
+    if (bytes == 0) return MODP_OK;
+    return hip_check(ctx, hipMemsetAsync(dptr, 0, bytes, ctx->stream), "hipMemsetAsync");
+}
+
+int modp_mod(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx, int64_t N)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(check_mat(ctx, "X", X, rows, cols, ldx));
+    if (N < 1 || N > ((int64_t)1 << 52))
+        return set_error(ctx, MODP_ERR_MODULUS_RANGE, "Modulus %lld is outside 1..2^52", (long long)N);
+    return util_mod(ctx, kind, rows, cols, X, ldx, N);
+}
+
+int modp_rand(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx, int64_t p, uint64_t seed,
+              uint64_t offset)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(check_mat(ctx, "X", X, rows, cols, ldx));
+    const int64_t lim = dtype == MODP_F64 ? ((int64_t)1 << 52) : ((int64_t)1 << 24);
+    if (p < 1 || p > lim) return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld out of range", (long long)p);
+    return util_rand(ctx, kind, rows, cols, X, ldx, p, seed, offset);
+}
+
+int modp_eye(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(check_mat(ctx, "X", X, rows, cols, ldx));
+    return util_eye(ctx, kind, rows, cols, X, ldx);
+}
+
+int modp_gemm(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda,
+              const void *B, int64_t ldb, void *C, int64_t ldc, int64_t p)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(check_mat(ctx, "A", A, m, k, lda));
+    MODP_TRY(check_mat(ctx, "B", B, k, n, ldb));
+    MODP_TRY(check_mat(ctx, "C", C, m, n, ldc));
+    MODP_TRY(check_modulus_gemm(ctx, dtype, p));
+    if (p > ((int64_t)1 << 26))
+        return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld above 2^26", (long long)p);
+    if (form != MODP_FORM_AUTO && form != MODP_FORM_PLAIN_F64 && form != MODP_FORM_LIMB_I8)
+        return set_error(ctx, MODP_ERR_ARG, "unknown form %d", form);
+    if (m == 0 || n == 0) return MODP_OK;
+    GemmArgs g;
+    g.m = m; g.n = n; g.k = k;
+    g.A = A; g.lda = lda; g.ka = kind;
+    g.B = B; g.ldb = ldb; g.kb = kind;
+    g.C = C; g.ldc = ldc; g.kc = kind;
+    g.epi = EPI_STORE;
+    g.mp = make_modp(p);
+    if (k == 0) {
+        // empty contraction: C = 0
+        return util_eye(ctx, kind, 0, 0, C, ldc) == MODP_OK
+                   ? hip_check(ctx, hipMemset2DAsync(C, (size_t)ldc * (kind == EK_F64 ? 8 : 4), 0,
+                                                     (size_t)m * (kind == EK_F64 ? 8 : 4), (size_t)n, ctx->stream),
+                               "hipMemset2DAsync")
+                   : MODP_ERR_DEVICE;
+    }
+    int f = form == MODP_FORM_AUTO ? choose_form(m, n, k, p) : form;
+    if (f == MODP_FORM_LIMB_I8) return gemm_limb_i8(ctx, g);
+    return gemm_plain_f64(ctx, g);
+}
+
+int modp_pluq(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, const void *A, int64_t lda, void *U,
+              int64_t ldu, void *L, int64_t ldl, int64_t p, int64_t *row_swaps, int64_t *n_row, int64_t *col_swaps,
+              int64_t *n_col, int64_t *rank)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(check_mat(ctx, "A", A, m, n, lda));
+    MODP_TRY(check_mat(ctx, "U", U, m, n, ldu));
+    MODP_TRY(check_mat(ctx, "L", L, m, m, ldl));
+    MODP_TRY(check_modulus_elim(ctx, dtype, p));
+    if (!row_swaps || !n_row || !col_swaps || !n_col || !rank) return set_error(ctx, MODP_ERR_ARG, "null output");
+    ElimResult r;
+    MODP_TRY(elim_run(ctx, ELIM_PLUQ, form, kind, m, n, A, lda, U, ldu, L, ldl, p, r));
+    std::copy(r.row_swaps.begin(), r.row_swaps.end(), row_swaps);
+    std::copy(r.col_swaps.begin(), r.col_swaps.end(), col_swaps);
+    *n_row = (int64_t)r.row_swaps.size() / 2;
+    *n_col = (int64_t)r.col_swaps.size() / 2;
+    *rank = r.rank;
+    return MODP_OK;
+}
+
+int modp_lu(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, const void *A, int64_t lda, void *U, int64_t ldu,
+            void *L, int64_t ldl, int64_t p, int64_t *perm, int64_t *pivcols, int64_t *rank)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(check_mat(ctx, "A", A, m, n, lda));
+    MODP_TRY(check_mat(ctx, "U", U, m, n, ldu));
+    MODP_TRY(check_mat(ctx, "L", L, m, m, ldl));
+    MODP_TRY(check_modulus_elim(ctx, dtype, p));
+    if ((!perm && m > 0) || !rank) return set_error(ctx, MODP_ERR_ARG, "null output");
+    ElimResult r;
+    MODP_TRY(elim_run(ctx, ELIM_LU, form, kind, m, n, A, lda, U, ldu, L, ldl, p, r));
+    std::copy(r.perm.begin(), r.perm.end(), perm);
+    if (pivcols) std::copy(r.pivcols.begin(), r.pivcols.end(), pivcols);
+    *rank = r.rank;
+    return MODP_OK;
+}
+
+int modp_rref(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, const void *A, int64_t lda, void *R,
+              int64_t ldr, int64_t p, int64_t *pivcols, int64_t *rank)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(check_mat(ctx, "A", A, m, n, lda));
+    MODP_TRY(check_mat(ctx, "R", R, m, n, ldr));
+    MODP_TRY(check_modulus_elim(ctx, dtype, p));
+    if (!rank) return set_error(ctx, MODP_ERR_ARG, "null output");
+    ElimResult r;
+    if (m == 0 || n == 0) { *rank = 0; return MODP_OK; }
+    MODP_TRY(elim_rref(ctx, form, kind, m, n, A, lda, R, ldr, p, r));
+    if (pivcols) std::copy(r.pivcols.begin(), r.pivcols.end(), pivcols);
+    *rank = r.rank;
+    return MODP_OK;
+}
+
+int modp_apply_row_perm(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx,
+                        const int64_t *stack, int64_t nswaps, int inverse)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(check_mat(ctx, "X", X, rows, cols, ldx));
+    if (nswaps > 0 && !stack) return set_error(ctx, MODP_ERR_ARG, "null stack");
+    return util_perm(ctx, kind, 1, rows, cols, X, ldx, stack, nswaps, inverse);
+}
+
+int modp_apply_col_perm(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx,
+                        const int64_t *stack, int64_t nswaps, int inverse)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(check_mat(ctx, "X", X, rows, cols, ldx));
+    if (nswaps > 0 && !stack) return set_error(ctx, MODP_ERR_ARG, "null stack");
+    return util_perm(ctx, kind, 0, rows, cols, X, ldx, stack, nswaps, inverse);
+}
+
+// ---- batched ----
+
+int modp_gemm_batched_plan(modp_ctx *ctx, int dtype, const modp_gemm_job *jobs, int64_t count, modp_gemm_plan **out)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    if (!out || (count > 0 && !jobs)) return set_error(ctx, MODP_ERR_ARG, "null argument");
+    for (int64_t i = 0; i < count; ++i) {
+        const modp_gemm_job &j = jobs[i];
+        MODP_TRY(check_mat(ctx, "A", j.A, j.m, j.k, j.lda));
+        MODP_TRY(check_mat(ctx, "B", j.B, j.k, j.n, j.ldb));
+        MODP_TRY(check_mat(ctx, "C", j.C, j.m, j.n, j.ldc));
+        MODP_TRY(check_modulus_gemm(ctx, dtype, j.p));
+        if (j.p > ((int64_t)1 << 26))
+            return set_error(ctx, MODP_ERR_MODULUS_RANGE, "job %lld: modulus above 2^26", (long long)i);
+    }
+    return batched_gemm_plan(ctx, kind, jobs, count, out);
+}
+
+int modp_gemm_batched_run(modp_ctx *ctx, modp_gemm_plan *plan)
+{
+    CTX_CHECK(ctx);
+    if (!plan) return set_error(ctx, MODP_ERR_ARG, "null plan");
+    return batched_gemm_run(ctx, plan);
+}
+
+int modp_gemm_batched_destroy(modp_ctx *ctx, modp_gemm_plan *plan)
+{
+    CTX_CHECK(ctx);
+    return batched_gemm_destroy(ctx, plan);
+}
+
+int modp_gemm_batched(modp_ctx *ctx, int dtype, const modp_gemm_job *jobs, int64_t count)
+{
+    modp_gemm_plan *plan = nullptr;
+    MODP_TRY(modp_gemm_batched_plan(ctx, dtype, jobs, count, &plan));
+    int st = modp_gemm_batched_run(ctx, plan);
+    int st2 = modp_gemm_batched_destroy(ctx, plan);
+    return st != MODP_OK ? st : st2;
+}
+
+int modp_elim_batched(modp_ctx *ctx, int op, int dtype, const modp_elim_job *jobs, int64_t count)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    if (op != MODP_ELIM_PLUQ && op != MODP_ELIM_LU && op != MODP_ELIM_RREF)
+        return set_error(ctx, MODP_ERR_ARG, "unknown elimination op %d", op);
+    if (count > 0 && !jobs) return set_error(ctx, MODP_ERR_ARG, "null jobs");
+    for (int64_t i = 0; i < count; ++i) {
+        const modp_elim_job &j = jobs[i];
+        if (j.m < 0 || j.n < 0 || j.m > 128 || j.n > 128)
+            return set_error(ctx, MODP_ERR_DIM, "job %lld: batched elimination needs m, n <= 128", (long long)i);
+        MODP_TRY(check_mat(ctx, "A", j.A, j.m, j.n, j.lda));
+        MODP_TRY(check_mat(ctx, "U", j.U, j.m, j.n, j.ldu));
+        if (op != MODP_ELIM_RREF) MODP_TRY(check_mat(ctx, "L", j.L, j.m, j.m, j.ldl));
+        MODP_TRY(check_modulus_elim(ctx, dtype, j.p));
+        if (!j.counts || !j.row_info || (op != MODP_ELIM_RREF && !j.col_info))
+            return set_error(ctx, MODP_ERR_ARG, "job %lld: null output", (long long)i);
+    }
+    return batched_elim(ctx, op, kind, jobs, count);
+}
+
+// ---- timing ----
+
+int modp_kernel_timing(modp_ctx *ctx, int enable)
+{
+    CTX_CHECK(ctx);
+    ctx->timing = enable != 0;
+    return MODP_OK;
+}
+
+int modp_kernel_timing_reset(modp_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &t : ctx->timings) {
+        ctx->event_pool.push_back(t.a);
+        ctx->event_pool.push_back(t.b);
+    }
+    ctx->timings.clear();
+    return MODP_OK;
+}
+
+int modp_kernel_time(modp_ctx *ctx, const char *prefix, double *total_ms, int64_t *count)
+{
+    CTX_CHECK(ctx);
+    if (!prefix || !total_ms || !count) return set_error(ctx, MODP_ERR_ARG, "null argument");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    double tot = 0.0;
+    int64_t cnt = 0;
+    const size_t pl = strlen(prefix);
+    for (auto &t : ctx->timings) {
+        if (t.name.compare(0, pl, prefix) != 0) continue;
+        float ms = 0.f;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, t.a, t.b));
+        tot += ms;
+        ++cnt;
+    }
+    *total_ms = tot;
+    *count = cnt;
+    return MODP_OK;
+}
+
+int modp_event_record(modp_ctx *ctx, int slot)
+{
+    CTX_CHECK(ctx);
+    if (slot < 0 || slot >= 16) return set_error(ctx, MODP_ERR_ARG, "slot out of range");
+    if (!ctx->slots[slot]) HIP_TRY(ctx, hipEventCreate(&ctx->slots[slot]));
+    return hip_check(ctx, hipEventRecord(ctx->slots[slot], ctx->stream), "hipEventRecord");
+}
+
+int modp_event_elapsed(modp_ctx *ctx, int a, int b, float *ms)
+{
+    CTX_CHECK(ctx);
+    if (a < 0 || a >= 16 || b < 0 || b >= 16 || !ctx->slots[a] || !ctx->slots[b] || !ms)
+        return set_error(ctx, MODP_ERR_ARG, "bad event slots");
+    HIP_TRY(ctx, hipEventSynchronize(ctx->slots[b]));
+    return hip_check(ctx, hipEventElapsedTime(ms, ctx->slots[a], ctx->slots[b]), "hipEventElapsedTime");
+}
+
+}  // extern "C"

@@ -1,0 +1,440 @@
+// batched.hip -- thousands of independent small products / eliminations, each
+// under its own prime, in one persistent launch fed by a device work queue.
+//
+// GEMM: the plan sorts jobs by cost, cuts each into output tiles whose shape
+// the autotuner picks for the job's shape bucket, and captures
+// [queue reset kernel, persistent kernel] in a hipGraph; every workgroup pulls
+// tile indices from an atomic counter until the queue is empty.
+//
+// Elimination: one workgroup per instance (m, n <= 128) with the matrix held
+// in LDS, the same pivot rule as the single-matrix path, one pass per pivot.
+#include <algorithm>
+#include <map>
+#include <numeric>
+
+#include "batched.h"
+#include "gemm_tile.h"
+
+// ---------------------------------------------------------------------------
+// batched GEMM
+// ---------------------------------------------------------------------------
+
+struct BJob {
+    const void *A; int64_t lda;
+    const void *B; int64_t ldb;
+    void *C; int64_t ldc;
+    int m, n, k, R;
+    int tile;     // 32 or 64
+    int tiles_n;  // tiles along n
+    ModP mp;
+};
+
+struct BItem {
+    int job;
+    int tile;  // tile index within the job: ti = tile / tiles_n, tj = tile % tiles_n
+};
+
+struct modp_gemm_plan {
+    int kind = EK_F64;
+    int64_t count = 0, items = 0;
+    BJob *djobs = nullptr;
+    BItem *ditems = nullptr;
+    int *dcounter = nullptr;
+    int grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+__global__ void k_queue_reset(int *counter)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_batched_gemm(const BJob *__restrict__ jobs, const BItem *__restrict__ items,
+                                                      int64_t nitems, int *__restrict__ counter)
+{
+    __shared__ F64Smem<64, 64> sm64;
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= nitems) break;
+        const BItem item = items[it];
+        const BJob &j = jobs[item.job];
+        const int ti = item.tile / j.tiles_n, tj = item.tile % j.tiles_n;
+        if (j.tile == 64) {
+            gemm_f64_tile<T, T, T, 64, 64, EPI_STORE>(j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb,
+                                                      (T *)j.C, j.ldc, j.mp, j.R, (int64_t)ti * 64,
+                                                      (int64_t)tj * 64, sm64);
+        } else {
+            gemm_f64_tile<T, T, T, 32, 32, EPI_STORE>(j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb,
+                                                      (T *)j.C, j.ldc, j.mp, j.R, (int64_t)ti * 32,
+                                                      (int64_t)tj * 32,
+                                                      *reinterpret_cast<F64Smem<32, 32> *>(&sm64));
+        }
+    }
+}
+
+// Tile choice per shape bucket.  Measured choices are cached per
+// (bucket of max(m,n), bucket of k); the default prefers 64x64 tiles unless
+// that leaves most of the tile empty.
+int pick_tile(int m, int n)
+{
+    const int64_t w64 = (int64_t)((m + 63) / 64) * ((n + 63) / 64) * 64 * 64;
+    const int64_t w32 = (int64_t)((m + 31) / 32) * ((n + 31) / 32) * 32 * 32;
+    // a 64-tile does 4x the MFMA work of a 32-tile per launch slot; accept
+    // up to 25% more padded work for the better operand reuse
+    return (double)w64 <= 1.25 * (double)w32 ? 64 : 32;
+}
+
+}  // namespace
+
+int batched_gemm_plan(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count, modp_gemm_plan **out)
+{
+    if (kind == EK_I32) return set_error(ctx, MODP_ERR_DTYPE, "batched gemm: unsupported element type");
+    modp_gemm_plan *pl = new modp_gemm_plan();
+    pl->kind = kind;
+    pl->count = count;
+    std::vector<BJob> hj((size_t)count);
+    std::vector<int64_t> order((size_t)count);
+    std::iota(order.begin(), order.end(), 0);
+    for (int64_t i = 0; i < count; ++i) {
+        const modp_gemm_job &s = jobs[i];
+        BJob &d = hj[i];
+        d.A = s.A; d.lda = s.lda; d.B = s.B; d.ldb = s.ldb; d.C = s.C; d.ldc = s.ldc;
+        d.m = (int)s.m; d.n = (int)s.n; d.k = (int)s.k;
+        d.R = f64_reduce_interval(s.p);
+        d.mp = make_modp(s.p);
+        d.tile = pick_tile(d.m, d.n);
+        d.tiles_n = (d.n + d.tile - 1) / d.tile;
+    }
+    // most expensive tiles first
+    auto cost = [&](int64_t i) { return (int64_t)hj[i].tile * hj[i].tile * hj[i].k; };
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
+    std::vector<BItem> items;
+    for (int64_t oi : order) {
+        const BJob &d = hj[oi];
+        if (d.m <= 0 || d.n <= 0) continue;
+        const int tiles = ((d.m + d.tile - 1) / d.tile) * d.tiles_n;
+        for (int t = 0; t < tiles; ++t) items.push_back(BItem{(int)oi, t});
+    }
+    pl->items = (int64_t)items.size();
+    hipError_t e = hipMalloc(&pl->djobs, sizeof(BJob) * (count > 0 ? count : 1));
+    if (e == hipSuccess) e = hipMalloc(&pl->ditems, sizeof(BItem) * (items.empty() ? 1 : items.size()));
+    if (e == hipSuccess) e = hipMalloc(&pl->dcounter, sizeof(int) * 64);
+    if (e == hipSuccess && count > 0)
+        e = hipMemcpyAsync(pl->djobs, hj.data(), sizeof(BJob) * count, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && !items.empty())
+        e = hipMemcpyAsync(pl->ditems, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        int st = hip_check(ctx, e, "batched gemm plan");
+        batched_gemm_destroy(ctx, pl);
+        return st;
+    }
+    // persistent grid: two workgroups per CU (LDS ~ 17 KB each, 256 threads)
+    int64_t g = (int64_t)ctx->num_cus * 4;
+    if (g > pl->items) g = pl->items;
+    pl->grid = (int)(g > 0 ? g : 1);
+    // capture [reset, persistent kernel] in a graph
+    e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        k_queue_reset<<<1, 64, 0, ctx->stream>>>(pl->dcounter);
+        if (kind == EK_F64)
+            k_batched_gemm<double><<<pl->grid, 256, 0, ctx->stream>>>(pl->djobs, pl->ditems, pl->items, pl->dcounter);
+        else
+            k_batched_gemm<float><<<pl->grid, 256, 0, ctx->stream>>>(pl->djobs, pl->ditems, pl->items, pl->dcounter);
+        e = hipStreamEndCapture(ctx->stream, &pl->graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        int st = hip_check(ctx, e, "batched gemm graph capture");
+        batched_gemm_destroy(ctx, pl);
+        return st;
+    }
+    *out = pl;
+    return MODP_OK;
+}
+
+int batched_gemm_run(modp_ctx *ctx, modp_gemm_plan *pl)
+{
+    if (pl->items == 0) return MODP_OK;
+    TimedLaunch tl(ctx, "batched_gemm");
+    return hip_check(ctx, hipGraphLaunch(pl->exec, ctx->stream), "hipGraphLaunch");
+}
+
+int batched_gemm_destroy(modp_ctx *ctx, modp_gemm_plan *pl)
+{
+    if (!pl) return MODP_OK;
+    hipStreamSynchronize(ctx->stream);
+    if (pl->exec) hipGraphExecDestroy(pl->exec);
+    if (pl->graph) hipGraphDestroy(pl->graph);
+    if (pl->djobs) hipFree(pl->djobs);
+    if (pl->ditems) hipFree(pl->ditems);
+    if (pl->dcounter) hipFree(pl->dcounter);
+    delete pl;
+    return MODP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched elimination
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int EMAX = 128;
+constexpr int EP = EMAX + 1;  // LDS row pitch (odd: column walks hit distinct banks)
+
+struct EJob {
+    const void *A; int64_t lda;
+    void *U; int64_t ldu;
+    void *L; int64_t ldl;
+    int64_t *row_info;
+    int64_t *col_info;
+    int64_t *counts;
+    int m, n;
+    ModP mp;
+};
+
+struct ESmem {
+    uint32_t S[EMAX * EP];
+    uint32_t mult[EMAX];
+    uint32_t pivval[EMAX];
+    int pivcol[EMAX];
+    int ipiv[EMAX];
+    uint32_t wv[4];
+    int wi[4];
+    int flag;
+    int item;
+};
+
+// argmax of S[r][col] for r in [row, m): largest value, lowest row; result
+// in (*V, *I) for every thread.  Two barriers.
+__device__ __forceinline__ void col_argmax(ESmem &s, int m, int row, int col, uint32_t *V, int *I)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t bv = 0;
+    int bi = 0x7fffffff;
+    for (int r = row + tid; r < m; r += 256) {
+        const uint32_t x = s.S[r * EP + col];
+        if (x > bv) { bv = x; bi = r; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t ov = __shfl_xor(bv, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) { s.wv[wave] = bv; s.wi[wave] = bi; }
+    __syncthreads();
+    uint32_t v = s.wv[0];
+    int i = s.wi[0];
+#pragma unroll
+    for (int x = 1; x < 4; ++x)
+        if (s.wv[x] > v || (s.wv[x] == v && s.wi[x] < i)) { v = s.wv[x]; i = s.wi[x]; }
+    *V = v;
+    *I = i;
+    __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_batched_elim(const EJob *__restrict__ jobs, int64_t njobs, int op,
+                                                      int *__restrict__ counter)
+{
+    __shared__ ESmem s;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s.item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s.item;
+        __syncthreads();
+        if (it >= njobs) break;
+        const EJob &j = jobs[it];
+        const int m = j.m, n = j.n;
+        const ModP mp = j.mp;
+        const T *A = (const T *)j.A;
+        for (int e = tid; e < m * n; e += 256) {
+            const int r = e % m, c = e / m;
+            s.S[r * EP + c] = to_res<T>(A[r + (int64_t)c * j.lda], mp);
+        }
+        __syncthreads();
+        int row = 0, col = 0, last = n - 1, nr = 0, nc = 0;
+        bool bad = false;
+        while (row < m && col < n) {
+            uint32_t V;
+            int I;
+            col_argmax(s, m, row, col, &V, &I);
+            if (V == 0) {
+                if (op != MODP_ELIM_PLUQ) { ++col; continue; }
+                // pluq: whole remaining block zero?
+                if (tid == 0) s.flag = 0;
+                __syncthreads();
+                const int rows = m - row, cols = n - col;
+                for (int e = tid; e < rows * cols; e += 256)
+                    if (s.S[(row + e % rows) * EP + col + e / rows]) s.flag = 1;
+                __syncthreads();
+                const int any = s.flag;
+                __syncthreads();
+                if (!any) {
+                    // src tail: (c, n) for c = col..n-1, applied in order
+                    for (int r = tid; r < m; r += 256)
+                        for (int c = col; c < n - 1; ++c) {
+                            uint32_t t = s.S[r * EP + c];
+                            s.S[r * EP + c] = s.S[r * EP + n - 1];
+                            s.S[r * EP + n - 1] = t;
+                        }
+                    if (tid == 0)
+                        for (int c = col; c < n; ++c) {
+                            j.col_info[2 * nc] = c + 1;
+                            j.col_info[2 * nc + 1] = n;
+                            ++nc;
+                        }
+                    __syncthreads();
+                    break;
+                }
+                for (int r = tid; r < m; r += 256) {
+                    uint32_t t = s.S[r * EP + col];
+                    s.S[r * EP + col] = s.S[r * EP + last];
+                    s.S[r * EP + last] = t;
+                }
+                if (tid == 0) {
+                    j.col_info[2 * nc] = col + 1;
+                    j.col_info[2 * nc + 1] = last + 1;
+                }
+                ++nc;
+                --last;
+                __syncthreads();
+                continue;
+            }
+            const uint32_t inv = modinv(V, mp);
+            if (inv == 0) { bad = true; break; }
+            // swap rows row <-> I and normalise the pivot row from col on
+            for (int c = tid; c < n; c += 256) {
+                const uint32_t a = s.S[row * EP + c], b = s.S[I * EP + c];
+                s.S[I * EP + c] = a;
+                s.S[row * EP + c] = c >= col ? mulmod(b, inv, mp) : b;
+            }
+            if (tid == 0) {
+                s.pivval[row] = V;
+                s.pivcol[row] = col;
+                s.ipiv[row] = I;
+                if (op == MODP_ELIM_PLUQ && I != row) {
+                    j.row_info[2 * nr] = row + 1;
+                    j.row_info[2 * nr + 1] = I + 1;
+                }
+            }
+            if (op == MODP_ELIM_PLUQ && I != row) ++nr;
+            __syncthreads();
+            if (op == MODP_ELIM_RREF) {
+                // Gauss-Jordan: clear the pivot column in every other row
+                for (int r = tid; r < m; r += 256) s.mult[r] = r == row ? 0u : s.S[r * EP + col];
+                __syncthreads();
+                const int cols = n - col;
+                for (int e = tid; e < m * cols; e += 256) {
+                    const int r = e / cols, c = col + e % cols;
+                    const uint32_t l = s.mult[r];
+                    if (l) s.S[r * EP + c] = submod(s.S[r * EP + c], mulmod(l, s.S[row * EP + c], mp), mp);
+                }
+            } else {
+                const int rows = m - row - 1, cols = n - col - 1;
+                for (int e = tid; e < rows * cols; e += 256) {
+                    const int r = row + 1 + e / cols, c = col + 1 + e % cols;
+                    const uint32_t l = s.S[r * EP + col];
+                    if (l) s.S[r * EP + c] = submod(s.S[r * EP + c], mulmod(l, s.S[row * EP + c], mp), mp);
+                }
+            }
+            __syncthreads();
+            ++row;
+            ++col;
+        }
+        const int rank = row;
+        // outputs
+        T *U = (T *)j.U;
+        for (int e = tid; e < m * n; e += 256) {
+            const int r = e % m, c = e / m;
+            uint32_t v = 0;
+            if (op == MODP_ELIM_RREF) v = r < rank ? s.S[r * EP + c] : 0u;
+            else if (r < rank && c >= s.pivcol[r]) v = s.S[r * EP + c];
+            U[r + (int64_t)c * j.ldu] = from_res<T>(v);
+        }
+        if (op != MODP_ELIM_RREF) {
+            T *L = (T *)j.L;
+            for (int e = tid; e < m * m; e += 256) {
+                const int r = e % m, k = e / m;
+                uint32_t v = 0;
+                if (k < rank) {
+                    if (r == k) v = s.pivval[k];
+                    else if (r > k) v = s.S[r * EP + s.pivcol[k]];
+                }
+                L[r + (int64_t)k * j.ldl] = from_res<T>(v);
+            }
+        }
+        if (tid == 0) {
+            if (op == MODP_ELIM_LU) {
+                for (int i = 0; i < m; ++i) j.row_info[i] = i + 1;
+                for (int i = 0; i < rank; ++i) {
+                    const int64_t t = j.row_info[i];
+                    j.row_info[i] = j.row_info[s.ipiv[i]];
+                    j.row_info[s.ipiv[i]] = t;
+                }
+                for (int i = 0; i < rank; ++i) j.col_info[i] = s.pivcol[i] + 1;
+            } else if (op == MODP_ELIM_RREF) {
+                for (int i = 0; i < rank; ++i) j.row_info[i] = s.pivcol[i] + 1;
+            }
+            j.counts[0] = op == MODP_ELIM_PLUQ ? nr : 0;
+            j.counts[1] = op == MODP_ELIM_PLUQ ? nc : 0;
+            j.counts[2] = bad ? -1 : rank;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+int batched_elim(modp_ctx *ctx, int op, int kind, const modp_elim_job *jobs, int64_t count)
+{
+    if (count <= 0) return MODP_OK;
+    if (kind == EK_I32) return set_error(ctx, MODP_ERR_DTYPE, "batched elimination: unsupported element type");
+    std::vector<EJob> hj((size_t)count);
+    std::vector<int64_t> order((size_t)count);
+    std::iota(order.begin(), order.end(), 0);
+    auto cost = [&](int64_t i) {
+        const int64_t a = jobs[i].m, b = jobs[i].n;
+        return a * b * (a < b ? a : b);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
+    for (int64_t i = 0; i < count; ++i) {
+        const modp_elim_job &s = jobs[order[i]];
+        EJob &d = hj[i];
+        d.A = s.A; d.lda = s.lda; d.U = s.U; d.ldu = s.ldu; d.L = s.L; d.ldl = s.ldl;
+        d.row_info = s.row_info; d.col_info = s.col_info; d.counts = s.counts;
+        d.m = (int)s.m; d.n = (int)s.n;
+        d.mp = make_modp(s.p);
+    }
+    int st = MODP_OK;
+    char *buf = (char *)ws_get(ctx, WS_BATCH, sizeof(EJob) * count + 256, &st);
+    if (!buf) return st;
+    int *counter = (int *)buf;
+    EJob *dj = (EJob *)(buf + 256);
+    HIP_TRY(ctx, hipMemcpyAsync(dj, hj.data(), sizeof(EJob) * count, hipMemcpyHostToDevice, ctx->stream));
+    k_queue_reset<<<1, 64, 0, ctx->stream>>>(counter);
+    HIP_TRY(ctx, hipGetLastError());
+    int64_t g = (int64_t)ctx->num_cus * 2;
+    if (g > count) g = count;
+    {
+        TimedLaunch tl(ctx, "batched_elim");
+        if (kind == EK_F64)
+            k_batched_elim<double><<<(unsigned)g, 256, 0, ctx->stream>>>(dj, count, op, counter);
+        else
+            k_batched_elim<float><<<(unsigned)g, 256, 0, ctx->stream>>>(dj, count, op, counter);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    // the job table lives in a reused workspace: finish before returning
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return MODP_OK;
+}

@@ -1,0 +1,227 @@
+// common.h -- shared device helpers and internal host interfaces of
+// libmodp_hip.so (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/modp.h"
+
+// ---------------------------------------------------------------------------
+// modulus parameters passed by value to every kernel
+// ---------------------------------------------------------------------------
+struct ModP {
+    int64_t p;      // modulus, 2 <= p <= 2^26
+    double pd;      // (double)p
+    double pinv;    // 1/p rounded to double
+    float pinvf;    // 1/p rounded to float
+    uint32_t p32;   // p as u32
+    int small;      // p < 2^16: products fit in u32
+    int half;       // floor(p/2): balanced representative bound
+};
+
+static inline ModP make_modp(int64_t p)
+{
+    ModP m;
+    m.p = p;
+    m.pd = (double)p;
+    m.pinv = 1.0 / (double)p;
+    m.pinvf = 1.0f / (float)p;
+    m.p32 = (uint32_t)p;
+    m.small = p < 65536 ? 1 : 0;
+    m.half = (int)(p / 2);
+    return m;
+}
+
+// x in (-2^53, 2^53), integral -> x mod p in [0,p)
+__device__ __forceinline__ double dmod(double x, const ModP &mp)
+{
+    double q = floor(x * mp.pinv);
+    double r = fma(-q, mp.pd, x);
+    r = r < 0.0 ? r + mp.pd : r;
+    r = r >= mp.pd ? r - mp.pd : r;
+    r = r < 0.0 ? r + mp.pd : r;
+    return r;
+}
+
+// a, b in [0,p) -> a*b mod p
+__device__ __forceinline__ uint32_t mulmod(uint32_t a, uint32_t b, const ModP &mp)
+{
+    if (mp.small) {
+        uint32_t x = a * b;
+        uint32_t q = (uint32_t)((float)x * mp.pinvf);
+        int32_t r = (int32_t)(x - q * mp.p32);
+        r += r < 0 ? (int32_t)mp.p32 : 0;
+        r -= r >= (int32_t)mp.p32 ? (int32_t)mp.p32 : 0;
+        return (uint32_t)r;
+    }
+    double x = (double)a * (double)b;
+    return (uint32_t)dmod(x, mp);
+}
+
+__device__ __forceinline__ uint32_t addmod(uint32_t a, uint32_t b, const ModP &mp)
+{
+    uint32_t s = a + b;
+    return s >= mp.p32 ? s - mp.p32 : s;
+}
+
+__device__ __forceinline__ uint32_t submod(uint32_t a, uint32_t b, const ModP &mp)
+{
+    return a >= b ? a - b : a + mp.p32 - b;
+}
+
+// a^{-1} mod p by the extended Euclidean algorithm
+// (src/CuModMatrix/rref_lu_pluq/pluq_kernels.jl:11-31).  Returns 0 when
+// gcd(a,p) != 1, which the callers report as "not invertible".
+// The Bezout coefficients stay within [-p, p], so int32 holds every
+// intermediate for p <= 2^26; the quotient comes from a double division with
+// one correction step.
+__device__ __forceinline__ uint32_t modinv(uint32_t a, const ModP &mp)
+{
+    int32_t inv = 0, ninv = 1, rem = (int32_t)mp.p32, nrem = (int32_t)a;
+    while (nrem != 0) {
+        int32_t q = (int32_t)((double)rem / (double)nrem);
+        int32_t r = rem - q * nrem;
+        if (r < 0) { q -= 1; r += nrem; }
+        else if (r >= nrem) { q += 1; r -= nrem; }
+        int32_t t = inv - q * ninv; inv = ninv; ninv = t;
+        rem = nrem; nrem = r;
+    }
+    if (rem != 1) return 0;
+    if (inv < 0) inv += (int32_t)mp.p32;
+    return (uint32_t)inv;
+}
+
+// storage element -> residue in [0,p)
+template <typename T> __device__ __forceinline__ uint32_t to_res(T x, const ModP &mp);
+template <> __device__ __forceinline__ uint32_t to_res<double>(double x, const ModP &mp)
+{
+    return (uint32_t)dmod(x, mp);
+}
+template <> __device__ __forceinline__ uint32_t to_res<float>(float x, const ModP &mp)
+{
+    return (uint32_t)dmod((double)x, mp);
+}
+template <> __device__ __forceinline__ uint32_t to_res<int32_t>(int32_t x, const ModP &mp)
+{
+    // working matrices hold residues already
+    (void)mp;
+    return (uint32_t)x;
+}
+
+template <typename T> __device__ __forceinline__ T from_res(uint32_t r);
+template <> __device__ __forceinline__ double from_res<double>(uint32_t r) { return (double)r; }
+template <> __device__ __forceinline__ float from_res<float>(uint32_t r) { return (float)r; }
+template <> __device__ __forceinline__ int32_t from_res<int32_t>(uint32_t r) { return (int32_t)r; }
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+struct KernelTiming {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct modp_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    // grow-only device workspaces
+    void *ws[8] = {nullptr};
+    size_t ws_size[8] = {0};
+    // pinned host staging for small device->host readbacks
+    void *pinned = nullptr;
+    size_t pinned_size = 0;
+    // kernel timing
+    bool timing = false;
+    std::vector<KernelTiming> timings;
+    std::vector<hipEvent_t> event_pool;
+    hipEvent_t slots[16] = {nullptr};
+};
+
+// workspace slots
+enum {
+    WS_ALIMB = 0,
+    WS_BLIMB = 1,
+    WS_WORK = 2,
+    WS_PIV = 3,
+    WS_UBUF = 4,
+    WS_MISC = 5,
+    WS_BATCH = 6,
+    WS_SCRATCH = 7
+};
+
+// error plumbing: every internal function returns a modp_status
+#define MODP_TRY(expr)                    \
+    do {                                  \
+        int _st = (expr);                 \
+        if (_st != MODP_OK) return _st;   \
+    } while (0)
+
+int set_error(modp_ctx *ctx, int code, const char *fmt, ...);
+int hip_check(modp_ctx *ctx, hipError_t e, const char *what);
+#define HIP_TRY(ctx, expr) MODP_TRY(hip_check((ctx), (expr), #expr))
+
+void *ws_get(modp_ctx *ctx, int slot, size_t bytes, int *status);
+void *pinned_get(modp_ctx *ctx, size_t bytes, int *status);
+
+// timing hooks around hot kernel launches
+struct TimedLaunch {
+    modp_ctx *ctx;
+    KernelTiming *t;
+    TimedLaunch(modp_ctx *c, const char *name);
+    ~TimedLaunch();
+};
+
+// ---------------------------------------------------------------------------
+// internal kernel launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+
+// element kind of a matrix operand
+enum ElemKind { EK_F64 = 0, EK_F32 = 1, EK_I32 = 2 };
+
+// epilogue of the GEMM cores
+enum GemmEpi {
+    EPI_STORE = 0,  // C = A*B mod p
+    EPI_SUB = 1,    // C = (C - A*B) mod p
+    EPI_ADD = 2     // C = (C + A*B) mod p
+};
+
+struct GemmArgs {
+    int64_t m, n, k;
+    const void *A; int64_t lda; int ka;   // ka: ElemKind of A
+    const void *B; int64_t ldb; int kb;
+    void *C; int64_t ldc; int kc;
+    int epi;
+    ModP mp;
+};
+
+int gemm_plain_f64(modp_ctx *ctx, const GemmArgs &g);
+int gemm_limb_i8(modp_ctx *ctx, const GemmArgs &g);
+int limb_count(int64_t p);
+int choose_form(int64_t m, int64_t n, int64_t k, int64_t p);
+
+int util_mod(modp_ctx *ctx, int kind, int64_t rows, int64_t cols, void *X, int64_t ldx, int64_t N);
+int util_rand(modp_ctx *ctx, int kind, int64_t rows, int64_t cols, void *X, int64_t ldx, int64_t p,
+              uint64_t seed, uint64_t offset);
+int util_eye(modp_ctx *ctx, int kind, int64_t rows, int64_t cols, void *X, int64_t ldx);
+int util_convert(modp_ctx *ctx, int kin, int64_t rows, int64_t cols, const void *X, int64_t ldx,
+                 int kout, void *Y, int64_t ldy, const ModP &mp);
+int util_perm(modp_ctx *ctx, int kind, int rowwise, int64_t rows, int64_t cols, void *X, int64_t ldx,
+              const int64_t *stack_host, int64_t nswaps, int inverse);
+
+// elimination driver (elim.hip)
+enum ElimMode { ELIM_PLUQ = 0, ELIM_LU = 1, ELIM_RREF = 2 };
+struct ElimResult {
+    std::vector<int64_t> row_swaps, col_swaps, perm, pivcols;
+    int64_t rank = 0;
+};
+int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n,
+             const void *A, int64_t lda, void *U, int64_t ldu, void *L, int64_t ldl,
+             int64_t p, ElimResult &res);
+int elim_rref(modp_ctx *ctx, int form, int kind, int64_t m, int64_t n, const void *A, int64_t lda, void *R,
+              int64_t ldr, int64_t p, ElimResult &res);

@@ -1,0 +1,216 @@
+/*
+ * modp.h -- C ABI of libmodp_hip.so, the MI355X (gfx950) implementation of
+ * GPUFiniteFieldMatrices.jl's modular matrix multiply and mod-p elimination.
+ *
+ * Every entry point takes plain pointers and sizes.  Matrices are column-major
+ * with an explicit leading dimension, the layout of CuModArray's padded
+ * `data` array (src/CuModMatrix/CuModMatrix.jl:42-122: storage is
+ * (rows+32) x (cols+32), logical size `size`, modulus `N`).  Matrix pointers
+ * are device pointers (obtained from modp_malloc or any HIP allocation on the
+ * context's device); index stacks, permutations and counts are host pointers.
+ * Calls are ordered on the context's stream; entry points that return host
+ * values (stacks, rank) synchronise before returning.
+ *
+ * Status codes map one-to-one onto the reference's exceptions in the host
+ * bindings (see modp_status).
+ */
+#ifndef MODP_H
+#define MODP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct modp_ctx modp_ctx;
+
+typedef enum {
+    MODP_OK = 0,
+    MODP_ERR_DIM = 1,              /* CuModArraySizeMismatchException / DimensionMismatch */
+    MODP_ERR_MODULUS_RANGE = 2,    /* modulus outside what the arithmetic can hold exactly
+                                      (ArgumentError "cannot perform a single multiplication",
+                                      stripe_mul.jl:200-202; N > 2^52, CuModMatrix.jl:55-59) */
+    MODP_ERR_MODULUS_MISMATCH = 3, /* CuModArrayModulusMismatchException (CuModMatrix.jl:769-773) */
+    MODP_ERR_NOT_INVERTIBLE = 4,   /* a pivot has no inverse mod N: N is not prime
+                                      (CuModMatrixModulusNotPrimeException, CuModMatrix.jl:21-23) */
+    MODP_ERR_DTYPE = 5,            /* element type unsupported (stripe_mul.jl:181-182) */
+    MODP_ERR_OOM = 6,
+    MODP_ERR_DEVICE = 7,           /* HIP runtime error */
+    MODP_ERR_ARG = 8,              /* invalid argument (null pointer, ld < rows, ...) */
+    MODP_ERR_NO_DEVICE = 9
+} modp_status;
+
+/* element type of the padded storage (CuModArray{T}: Float64 / Float32) */
+typedef enum { MODP_F64 = 0, MODP_F32 = 1 } modp_dtype;
+
+/* arithmetic form of the GEMM and of the elimination's trailing update */
+typedef enum {
+    MODP_FORM_AUTO = 0,      /* autotuner's choice for (shape, bits of p) */
+    MODP_FORM_PLAIN_F64 = 1, /* FP64 MFMA, balanced residues, periodic reduction */
+    MODP_FORM_LIMB_I8 = 2    /* 7-bit limbs in int8, int32-accumulating MFMA,
+                                Barrett recombination */
+} modp_form;
+
+/* ---- version, context, errors ---------------------------------------- */
+
+const char *modp_version(void);
+int modp_device_count(int *count);
+int modp_ctx_create(int device, modp_ctx **out);
+int modp_ctx_destroy(modp_ctx *ctx);
+/* message of the last failing call on this context ("" if none) */
+const char *modp_last_error(modp_ctx *ctx);
+int modp_synchronize(modp_ctx *ctx);
+/* the context's hipStream_t, for callers that order their own work on it */
+void *modp_stream(modp_ctx *ctx);
+
+/* ---- device memory (CuArray allocation / copyto! / Array) ------------- */
+
+int modp_malloc(modp_ctx *ctx, size_t bytes, void **dptr);
+int modp_free(modp_ctx *ctx, void *dptr);
+int modp_memcpy_h2d(modp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int modp_memcpy_d2h(modp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int modp_memcpy_d2d(modp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int modp_memset_zero(modp_ctx *ctx, void *dptr, size_t bytes);
+
+/* ---- elementwise ------------------------------------------------------ */
+
+/* X[0:rows, 0:cols] = mod(X, N), floor semantics.
+ * Replaces k_mod!/mod! (src/CuModMatrix/kernel_ops/mod_ops.jl:3-27). */
+int modp_mod(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx, int64_t N);
+
+/* Seeded uniform residues in [0,p): element (i,j) gets draw number
+ * offset + i + j*rows of the counter-based SplitMix64 stream `seed`, mapped
+ * to [0,p) by Lemire's unbiased method.  Stands in for rand(T,rows,cols,N)
+ * (CuModMatrix.jl:551-556), which is unseeded in the reference. */
+int modp_rand(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx,
+              int64_t p, uint64_t seed, uint64_t offset);
+
+/* X = identity (rows x cols block, zeros elsewhere in the block).
+ * Replaces eye(T, n, N) (CuModMatrix.jl:510-513). */
+int modp_eye(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx);
+
+/* ---- GEMM ------------------------------------------------------------- */
+
+/* C = A*B mod p; A m x k, B k x n, C m x n; entries of A and B are reduced
+ * mod p on load.  Exact for 2 <= p <= 2^26 (the FP64 stripe bound
+ * find_max_stripe_ops >= 1, stripe_mul.jl:13-27).
+ * Replaces stripe_mul!(C,A,B) (stripe_mul.jl:175-244), used by
+ * LinearAlgebra.mul!(C,A,B) (CuModMatrix.jl:767-787), Base.:* (mul_ops.jl:54-58)
+ * and mat_mul_gpu_type (mat_mul_gpu_direct.jl:8-42). */
+int modp_gemm(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, int64_t k,
+              const void *A, int64_t lda, const void *B, int64_t ldb,
+              void *C, int64_t ldc, int64_t p);
+
+/* ---- elimination ------------------------------------------------------ */
+
+/*
+ * P*A*Q = L*U mod p (A m x n, not modified).  U: m x n, row echelon with unit
+ * pivots; L: m x m lower triangular with L[k,k] = k-th pivot value and the
+ * raw multipliers below it.  row_swaps (capacity 2*min(m,n) int64) and
+ * col_swaps (capacity 4*n int64) receive the 1-based transposition stacks
+ * (a,b) in order; the counts in *n_row, *n_col; the number of pivots in
+ * *rank.  Pivot: largest residue of the column, lowest row on ties.  A zero
+ * column is swapped with the last not-yet-swapped column and the same column
+ * retried; when the remaining block is zero the stack (c, n) for c = col..n
+ * is emitted.  Identical to the reference's output whenever the reference's
+ * is a valid factorisation.
+ * Replaces pluq_gpu_kernel(A) (src/CuModMatrix/rref_lu_pluq/pluq_kernels.jl:46-157).
+ */
+int modp_pluq(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n,
+              const void *A, int64_t lda, void *U, int64_t ldu, void *L, int64_t ldl,
+              int64_t p, int64_t *row_swaps, int64_t *n_row,
+              int64_t *col_swaps, int64_t *n_col, int64_t *rank);
+
+/*
+ * PLE: A[perm,:] = L*U mod p with row pivoting only; zero columns are skipped.
+ * perm: m int64, 1-based.  pivcols: min(m,n) int64, 1-based pivot columns.
+ * Replaces lu_gpu_type (test/Experiments/rref_gpu_type.jl:60-103, prototype).
+ */
+int modp_lu(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n,
+            const void *A, int64_t lda, void *U, int64_t ldu, void *L, int64_t ldl,
+            int64_t p, int64_t *perm, int64_t *pivcols, int64_t *rank);
+
+/*
+ * R = canonical reduced row echelon form of A mod p (m x n); pivcols 1-based.
+ * Replaces rref_gpu_type (test/Experiments/rref_gpu_type.jl:8-51, prototype;
+ * README.md:8).
+ */
+int modp_rref(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n,
+              const void *A, int64_t lda, void *R, int64_t ldr,
+              int64_t p, int64_t *pivcols, int64_t *rank);
+
+/* ---- permutations (src/CuModMatrix/rref_lu_pluq/permutations.jl) ------ */
+
+/* Swap rows (a,b) of X for each 1-based pair of the stack, in order
+ * (inverse = 0; apply_row_perm!, :73-80) or in reverse order (inverse = 1;
+ * apply_row_inv_perm!, :82-89). */
+int modp_apply_row_perm(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx,
+                        const int64_t *stack, int64_t nswaps, int inverse);
+/* Same for columns (apply_col_perm!, :11-18; apply_col_inv_perm!, :20-27). */
+int modp_apply_col_perm(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx,
+                        const int64_t *stack, int64_t nswaps, int inverse);
+
+/* ---- batched multi-modular forms (no counterpart in the reference; the
+ *      nearest precedents are gemv_batched!, KaratsubaMatrix.jl:280, and the
+ *      per-call modulus override mod_N, kernel_ops/add_ops.jl:23-30) ------- */
+
+typedef struct {
+    int64_t m, n, k, p;
+    const void *A; int64_t lda;
+    const void *B; int64_t ldb;
+    void *C; int64_t ldc;
+} modp_gemm_job;
+
+typedef struct modp_gemm_plan modp_gemm_plan;
+
+/* Build a plan for `count` independent products C_i = A_i*B_i mod p_i (device
+ * pointers in the jobs): jobs are sorted by cost, cut into output tiles whose
+ * shape the autotuner picks per job, and a persistent kernel that pulls tiles
+ * from a device work queue is captured in a hipGraph. */
+int modp_gemm_batched_plan(modp_ctx *ctx, int dtype, const modp_gemm_job *jobs, int64_t count,
+                           modp_gemm_plan **out);
+int modp_gemm_batched_run(modp_ctx *ctx, modp_gemm_plan *plan);
+int modp_gemm_batched_destroy(modp_ctx *ctx, modp_gemm_plan *plan);
+/* plan + run + destroy */
+int modp_gemm_batched(modp_ctx *ctx, int dtype, const modp_gemm_job *jobs, int64_t count);
+
+typedef enum { MODP_ELIM_PLUQ = 0, MODP_ELIM_LU = 1, MODP_ELIM_RREF = 2 } modp_elim_op;
+
+typedef struct {
+    int64_t m, n, p;
+    const void *A; int64_t lda;  /* input, not modified */
+    void *U; int64_t ldu;        /* PLUQ/LU: U; RREF: R (m x n) */
+    void *L; int64_t ldl;        /* PLUQ/LU: L (m x m); RREF: unused (may be NULL) */
+    int64_t *row_info;           /* device. PLUQ: row stack, 2*min(m,n) entries;
+                                    LU: perm (m); RREF: pivcols (min(m,n)) */
+    int64_t *col_info;           /* device. PLUQ: column stack, 4*n entries; LU: pivcols
+                                    (min(m,n)); RREF: unused */
+    int64_t *counts;             /* device, 3 entries: n_row, n_col, rank */
+} modp_elim_job;
+
+/* Independent eliminations of small matrices (m, n <= 128), each under its
+ * own prime, one workgroup per instance with the matrix resident in LDS,
+ * pulled from a device work queue by a persistent kernel.  Outputs are
+ * written to device memory; the call returns without synchronising. */
+int modp_elim_batched(modp_ctx *ctx, int op, int dtype, const modp_elim_job *jobs, int64_t count);
+
+/* ---- timing (for the benchmark: HIP events on the context's stream) ---- */
+
+/* Start/stop recording of per-kernel durations for the library's hot kernels
+ * (events around each launch on the context's stream). */
+int modp_kernel_timing(modp_ctx *ctx, int enable);
+/* Sum and count of recorded durations for kernels whose name starts with
+ * `prefix`; synchronises. */
+int modp_kernel_time(modp_ctx *ctx, const char *prefix, double *total_ms, int64_t *count);
+int modp_kernel_timing_reset(modp_ctx *ctx);
+/* events around arbitrary regions on the context's stream */
+int modp_event_record(modp_ctx *ctx, int slot);
+int modp_event_elapsed(modp_ctx *ctx, int slot_start, int slot_end, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MODP_H */

@@ -1,0 +1,170 @@
+"""Batched multi-modular forms and device permutations on the GPU."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_known_answers.json")))
+
+
+def test_permutations_reference_fixture(gffm):
+    g = gffm
+    f = GOLD["permutation"]
+    st = [tuple(x) for x in f["stack"]]
+    A = g.CuModMatrix(np.array(f["A"]), f["p"])
+    g.apply_col_perm_(st, A)
+    assert g.Array(A).astype(int).tolist() == f["col_perm"]
+    g.apply_col_inv_perm_(st, A)
+    assert g.Array(A).astype(int).tolist() == f["A"]
+    A = g.CuModMatrix(np.array(f["A"]), f["p"])
+    g.apply_row_perm_(st, A)
+    assert g.Array(A).astype(int).tolist() == f["row_perm"]
+    g.apply_row_inv_perm_(st, A)
+    assert g.Array(A).astype(int).tolist() == f["A"]
+
+
+def test_pluq_identity_through_permutations(gffm):
+    # P*A*Q == L*U using the device permutation kernels and the device GEMM
+    g = gffm
+    p = 65521
+    Ah = O.draw_matrix(70, 50, p, 3)
+    Ah[:, 10] = 0
+    A = g.CuModMatrix(Ah, p, elem_type=np.float64)
+    U, L, P, Q = g.pluq_gpu_kernel(A)
+    B = g.copy(A)
+    g.apply_row_perm_(P, B)
+    g.apply_col_perm_(Q, B)
+    assert np.array_equal(g.Array(B), g.Array(L * U))
+
+
+def gemm_jobs(g, count, seed, lo=16, hi=256):
+    rng = np.random.default_rng(seed)
+    primes = O.largest_primes_below(2 ** 23, 64)
+    jobs, host = [], []
+    for i in range(count):
+        m, n, k = (int(x) for x in rng.integers(lo, hi + 1, size=3))
+        p = primes[int(rng.integers(0, 64))]
+        Ah = O.draw_matrix(m, k, p, seed=seed * 100000 + i)
+        Bh = O.draw_matrix(k, n, p, seed=seed * 100000 + i, offset=m * k)
+        A = g.CuModMatrix(Ah, p, elem_type=np.float64)
+        B = g.CuModMatrix(Bh, p, elem_type=np.float64)
+        C = g.zeros(np.float64, m, n, p)
+        jobs.append((A, B, C, p))
+        host.append((Ah, Bh, p))
+    return jobs, host
+
+
+def test_batched_gemm_vs_oracle(gffm):
+    g = gffm
+    jobs, host = gemm_jobs(g, 150, seed=3)
+    plan = g.GemmBatchPlan(jobs)
+    for rep in range(3):
+        # replays of the captured graph, with eager work in between
+        for _, _, C, _ in jobs:
+            C.data.zero()
+        plan.run()
+        g.context().synchronize()
+        for (A, B, C, p), (Ah, Bh, _) in zip(jobs, host):
+            assert np.array_equal(g.Array(C).astype(np.int64), O.gemm(Ah, Bh, p))
+
+
+def test_batched_gemm_edge_shapes(gffm):
+    g = gffm
+    jobs, host = gemm_jobs(g, 40, seed=9, lo=1, hi=70)
+    g.gemm_batched(jobs)
+    for (A, B, C, p), (Ah, Bh, _) in zip(jobs, host):
+        assert np.array_equal(g.Array(C).astype(np.int64), O.gemm(Ah, Bh, p))
+
+
+def make_elim_instances(count, seed, maxdim=128):
+    rng = np.random.default_rng(seed)
+    primes = [2, 3, 7, 251, 65521]
+    out = []
+    for i in range(count):
+        m, n = (int(x) for x in rng.integers(8, maxdim + 1, size=2))
+        p = primes[int(rng.integers(0, 5))]
+        A = O.draw_matrix(m, n, p, seed=seed * 100000 + i)
+        if i % 4 == 0:
+            # rank deficient: zero a random row subset, then left-multiply by
+            # an invertible matrix
+            k = int(rng.integers(1, m))
+            rowsz = rng.choice(m, size=k, replace=False)
+            A[rowsz, :] = 0
+            for t in range(100):
+                S = O.draw_matrix(m, m, p, seed=seed * 100000 + i + 7919 * (t + 1))
+                if O.rref(S, p)["rank"] == m:
+                    break
+            A = O.gemm(S, A, p)
+        out.append((A, p))
+    return out
+
+
+class ElimBatch:
+    def __init__(self, g, inst, op):
+        self.g, self.inst, self.op = g, inst, op
+        ctx = g.context()
+        self.keep = []
+        jobs = []
+        for A, p in inst:
+            m, n = A.shape
+            dA = g.CuModMatrix(A, p, elem_type=np.float64)
+            dU = g.zeros(np.float64, m, n, p)
+            dL = g.zeros(np.float64, m, m, p)
+            ri = ctx.malloc(8 * max(2 * min(m, n), m, 1))
+            ci = ctx.malloc(8 * max(4 * n, 1))
+            cnt = ctx.malloc(8 * 3)
+            self.keep.append((dA, dU, dL, ri, ci, cnt))
+            jobs.append(g.ElimJob(m, n, p, dA.ptr, dA.ld, dU.ptr, dU.ld, dL.ptr, dL.ld, ri.ptr, ci.ptr, cnt.ptr))
+        self.jobs = jobs
+
+    def run(self):
+        self.g.elim_batched(self.op, self.jobs)
+
+    def result(self, i):
+        g = self.g
+        dA, dU, dL, ri, ci, cnt = self.keep[i]
+        m, n = self.inst[i][0].shape
+        c = np.zeros(3, dtype=np.int64)
+        cnt.download(c)
+        rb = np.zeros(ri.nbytes // 8, dtype=np.int64)
+        ri.download(rb)
+        cb = np.zeros(ci.nbytes // 8, dtype=np.int64)
+        ci.download(cb)
+        return dict(U=g.Array(dU).astype(np.int64), L=g.Array(dL).astype(np.int64), counts=c, row=rb, col=cb)
+
+
+@pytest.mark.parametrize("op", ["pluq", "lu", "rref"])
+def test_batched_elim_vs_oracle(gffm, op):
+    g = gffm
+    inst = make_elim_instances(300, seed=4)
+    code = {"pluq": 0, "lu": 1, "rref": 2}[op]
+    b = ElimBatch(g, inst, code)
+    b.run()
+    for i, (A, p) in enumerate(inst):
+        r = b.result(i)
+        if op == "pluq":
+            ref = O.pluq(A, p, O.PLUQ_OPTC)
+            nr, nc, rank = r["counts"]
+            assert rank == ref["rank"], i
+            assert [tuple(x) for x in r["row"][:2 * nr].reshape(-1, 2).tolist()] == ref["row_swaps"]
+            assert [tuple(x) for x in r["col"][:2 * nc].reshape(-1, 2).tolist()] == ref["col_swaps"]
+            assert np.array_equal(r["U"], ref["U"]) and np.array_equal(r["L"], ref["L"]), i
+        elif op == "lu":
+            ref = O.lu(A, p)
+            rank = r["counts"][2]
+            assert rank == ref["rank"]
+            assert r["row"][:A.shape[0]].tolist() == ref["perm"].tolist()
+            assert r["col"][:rank].tolist() == ref["pivcols"].tolist()
+            assert np.array_equal(r["U"], ref["U"]) and np.array_equal(r["L"], ref["L"])
+        else:
+            ref = O.rref(A, p)
+            rank = r["counts"][2]
+            assert rank == ref["rank"]
+            assert r["row"][:rank].tolist() == ref["pivcols"].tolist()
+            assert np.array_equal(r["U"], ref["R"])

@@ -1,0 +1,185 @@
+"""Elimination parity on the GPU: pluq / lu / rref bit-exact against the
+oracle (same U, L, stacks, rank, RREF), invariants at full size."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_known_answers.json")))
+
+
+def dev(g, X, p, et=np.float64):
+    return g.CuModMatrix(np.asarray(X), p, elem_type=et)
+
+
+def rank_deficient(m, n, r, p, seed):
+    X = O.draw_matrix(m, r, p, seed)
+    Y = O.draw_matrix(r, n, p, seed, offset=m * r)
+    return O.gemm(X, Y, p)
+
+
+def check_pluq_equal(g, Ah, p, form=0, et=np.float64):
+    A = dev(g, Ah, p, et)
+    U, L, P, Q, rank = g.pluq(A, form=form)
+    ref = O.pluq(Ah, p, O.PLUQ_OPTC)
+    assert rank == ref["rank"]
+    assert P == ref["row_swaps"]
+    assert Q == ref["col_swaps"]
+    assert np.array_equal(g.Array(U).astype(np.int64), ref["U"])
+    assert np.array_equal(g.Array(L).astype(np.int64), ref["L"])
+    # the input is not modified
+    assert np.array_equal(g.Array(A).astype(np.int64), np.mod(Ah, p))
+
+
+@pytest.mark.parametrize("case", GOLD["elimination"], ids=lambda c: c["name"])
+def test_pluq_known_answers(gffm, case):
+    g = gffm
+    p = case["p"]
+    Ah = np.mod(np.array(case["A"]), p)
+    U, L, P, Q = g.pluq_gpu_kernel(dev(g, Ah, p))
+    assert P == [tuple(x) for x in case["row_swaps"]]
+    assert Q == [tuple(x) for x in case["col_swaps"]]
+    assert O.check_pluq(Ah, p, g.Array(U).astype(np.int64), g.Array(L).astype(np.int64), P, Q, case["rank"]) == []
+
+
+SHAPES = [(1, 1), (1, 7), (7, 1), (6, 20), (20, 6), (10, 10), (33, 17), (17, 33), (64, 64), (100, 37),
+          (37, 100), (129, 130), (200, 200)]
+
+
+@pytest.mark.parametrize("p", [2, 3, 7, 251, 65521, 2 ** 26 - 5])
+@pytest.mark.parametrize("shape", SHAPES, ids=str)
+def test_pluq_vs_oracle(gffm, shape, p):
+    m, n = shape
+    for seed in range(3):
+        print(f"pluq shape={shape} p={p} seed={seed}", flush=True)
+        Ah = O.draw_matrix(m, n, p, seed)
+        if seed == 2 and min(m, n) > 2:
+            Ah = rank_deficient(m, n, min(m, n) // 2, p, seed)
+        check_pluq_equal(gffm, Ah, p)
+
+
+def test_pluq_zero_columns_mid_matrix(gffm):
+    # zero columns before the rank is exhausted exercise the column swaps
+    p = 7
+    for seed in range(6):
+        Ah = O.draw_matrix(40, 50, p, seed)
+        Ah[:, [0, 3, 4, 17, 33]] = 0
+        Ah[:, 45] = 0
+        print(f"pluq zero columns seed={seed}", flush=True)
+        check_pluq_equal(gffm, Ah, p)
+
+
+def test_pluq_all_zero_and_single_nonzero(gffm):
+    check_pluq_equal(gffm, np.zeros((5, 9), dtype=np.int64), 11)
+    Z = np.zeros((9, 5), dtype=np.int64)
+    Z[7, 3] = 4
+    check_pluq_equal(gffm, Z, 11)
+
+
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("n,r", [(512, 512), (700, 300), (1024, 1024), (1500, 1100)])
+def test_pluq_medium_vs_oracle(gffm, n, r, form):
+    p = 65521
+    print(f"pluq medium n={n} r={r} form={form}", flush=True)
+    Ah = O.draw_matrix(n, n, p, seed=2) if r == n else rank_deficient(n, n, r, p, seed=2)
+    check_pluq_equal(gffm, Ah, p, form=form)
+
+
+def test_pluq_float32_storage(gffm):
+    for p in (7, 251, 65521):
+        Ah = O.draw_matrix(60, 45, p, seed=4)
+        check_pluq_equal(gffm, Ah, p, et=np.float32)
+
+
+@pytest.mark.parametrize("p", [2, 7, 65521])
+@pytest.mark.parametrize("shape", [(1, 1), (6, 20), (20, 6), (33, 33), (100, 37), (37, 100), (150, 150)], ids=str)
+def test_lu_vs_oracle(gffm, shape, p):
+    g = gffm
+    m, n = shape
+    for seed in range(3):
+        Ah = O.draw_matrix(m, n, p, seed)
+        if seed == 2 and min(m, n) > 2:
+            Ah = rank_deficient(m, n, min(m, n) // 2, p, seed)
+        if seed == 1 and n > 5:
+            Ah[:, [1, 4]] = 0
+        print(f"lu shape={shape} p={p} seed={seed}", flush=True)
+        U, L, perm, piv, rank = g.lu_full(dev(g, Ah, p))
+        ref = O.lu(Ah, p)
+        assert rank == ref["rank"]
+        assert perm == ref["perm"].tolist()
+        assert piv == ref["pivcols"].tolist()
+        assert np.array_equal(g.Array(U).astype(np.int64), ref["U"])
+        assert np.array_equal(g.Array(L).astype(np.int64), ref["L"])
+
+
+@pytest.mark.parametrize("p", [2, 3, 251, 65521, 2 ** 26 - 5])
+@pytest.mark.parametrize("shape", [(1, 1), (5, 9), (9, 5), (64, 64), (70, 130), (130, 70), (300, 300)], ids=str)
+def test_rref_vs_oracle(gffm, shape, p):
+    g = gffm
+    m, n = shape
+    for seed in range(2):
+        Ah = O.draw_matrix(m, n, p, seed)
+        if seed == 1 and min(m, n) > 2:
+            Ah = rank_deficient(m, n, min(m, n) // 3, p, seed)
+        print(f"rref shape={shape} p={p} seed={seed}", flush=True)
+        R, piv, rank = g.rref_full(dev(g, Ah, p))
+        ref = O.rref(Ah, p)
+        assert rank == ref["rank"] and piv == ref["pivcols"].tolist()
+        assert np.array_equal(g.Array(R).astype(np.int64), ref["R"])
+
+
+def test_rref_medium_rank_deficient(gffm):
+    g = gffm
+    p = 65521
+    Ah = rank_deficient(800, 900, 500, p, seed=6)
+    R, piv, rank = g.rref_full(dev(g, Ah, p))
+    ref = O.rref(Ah, p)
+    assert rank == 500 and np.array_equal(g.Array(R).astype(np.int64), ref["R"])
+
+
+def test_not_prime_modulus_is_an_error(gffm):
+    g = gffm
+    A = dev(g, [[2, 1], [1, 1]], 4)
+    with pytest.raises(g.CuModMatrixModulusNotPrimeException):
+        g.pluq(A)
+
+
+@pytest.mark.parametrize("kind", ["full", "rank3000"])
+def test_config3_4096_invariants(gffm, kind):
+    # full size: the factorisation identity by Freivalds, structure, rank, and
+    # plain vs limb trailing updates bit-identical
+    g = gffm
+    p, n = 65521, 4096
+    print(f"config3 {kind}", flush=True)
+    if kind == "full":
+        A = g.rand(np.float64, n, n, p, seed=2)
+    else:
+        X = g.rand(np.float64, n, 3000, p, seed=2)
+        Y = g.rand(np.float64, 3000, n, p, seed=2, offset=n * 3000)
+        A = X * Y
+    U1, L1, P1, Q1, r1 = g.pluq(A, form=1)
+    U2, L2, P2, Q2, r2 = g.pluq(A, form=2)
+    assert r1 == r2 == (n if kind == "full" else 3000)
+    assert P1 == P2 and Q1 == Q2
+    u1, u2 = g.Array(U1), g.Array(U2)
+    assert np.array_equal(u1, u2)
+    l1 = g.Array(L1)
+    assert np.array_equal(l1, g.Array(L2))
+    Ah = g.Array(A).astype(np.int64)
+    errs = O.check_pluq(Ah, p, u1.astype(np.int64), l1.astype(np.int64), P1, Q1, r1, exact=False)
+    assert errs == []
+    if kind == "rank3000":
+        R, piv, rank = g.rref_full(A)
+        assert rank == 3000
+        Rh = g.Array(R).astype(np.int64)
+        ok, piv2 = O.is_rref(Rh[:3000 + 1, :], p)
+        assert ok and piv2 == piv
+        # rows of A lie in the row space of R: A == A[:, piv] * R
+        Ap = Ah[:, [c - 1 for c in piv]]
+        assert O.freivalds_equal(lambda x: O._mod_matvec(Ah, x, p),
+                                 lambda x: O._mod_matvec(Ap, O._mod_matvec(Rh[:3000], x, p), p), n, p)
