@@ -233,17 +233,19 @@ __global__ __launch_bounds__(1024) void k_panel(int32_t *__restrict__ W, int64_t
     }
 }
 
-// One wave per column outside the panel [c0, c0+w): apply the panel's row
-// swaps; for columns >= solve_from also solve with the panel's L11 and write
-// the rows into Ubig (rows indexed by pivot column - KC, columns by c - KC).
-__global__ __launch_bounds__(256) void k_finish(int32_t *__restrict__ W, int64_t ldw, int64_t n, int kr0, int c0,
-                                                int w, int KC, const PanelCtl *__restrict__ ctl,
+// One wave per column of the outer block [ca, cb) outside the panel
+// [c0, c0+w): apply the panel's row swaps (a gather map whose first npiv
+// entries are the pivot rows in order); for columns right of the panel also
+// solve with the panel's L11 and write the U rows into Ubig (rows indexed by
+// pivot column - KC, columns by c - KC).
+__global__ __launch_bounds__(256) void k_finish(int32_t *__restrict__ W, int64_t ldw, int64_t ca, int64_t cb,
+                                                int kr0, int c0, int w, int KC, const PanelCtl *__restrict__ ctl,
                                                 const int32_t *__restrict__ pivinv,
                                                 const int32_t *__restrict__ pivcol,
                                                 const int32_t *__restrict__ colmap,
                                                 const int32_t *__restrict__ map_dst,
                                                 const int32_t *__restrict__ map_src, int32_t *__restrict__ Ubig,
-                                                int64_t ldub, int64_t solve_from, ModP mp)
+                                                int64_t ldub, ModP mp)
 {
     __shared__ uint32_t Ls[64][65];
     __shared__ uint32_t invs[64];
@@ -259,23 +261,20 @@ __global__ __launch_bounds__(256) void k_finish(int32_t *__restrict__ W, int64_t
         pcols[tid] = pivcol[kr0 + tid];
     }
     __syncthreads();
-    const int64_t ncols = n - w;
+    const int64_t ncols = (cb - ca) - w;
     const int d0 = lane < nmap ? map_dst[lane] : 0, s0 = lane < nmap ? map_src[lane] : 0;
-    const int d1 = lane + 64 < nmap ? map_dst[lane + 64] : 0, s1 = lane + 64 < nmap ? map_src[lane + 64] : 0;
     const bool zero_row = lane < w && colmap[lane] < 0;
     for (int64_t ci = (int64_t)blockIdx.x * 4 + wave; ci < ncols; ci += (int64_t)gridDim.x * 4) {
-        const int64_t c = ci < c0 ? ci : ci + w;
+        int64_t c = ca + ci;
+        if (c >= c0) c += w;
         int32_t *col = W + c * ldw;
-        uint32_t v0 = 0, v1 = 0;
+        uint32_t v0 = 0;
         if (lane < nmap) v0 = (uint32_t)col[s0];
-        if (lane + 64 < nmap) v1 = (uint32_t)col[s1];
-        if (c < solve_from) {
+        if (c < c0) {
             if (lane < nmap) col[d0] = (int32_t)v0;
-            if (lane + 64 < nmap) col[d1] = (int32_t)v1;
             continue;
         }
         if (lane >= npiv && lane < nmap) col[d0] = (int32_t)v0;
-        if (lane + 64 < nmap) col[d1] = (int32_t)v1;
         uint32_t x = v0;
         for (int j = 0; j < npiv; ++j) {
             if (lane == j) x = mulmod(x, invs[j], mp);
@@ -288,6 +287,80 @@ __global__ __launch_bounds__(256) void k_finish(int32_t *__restrict__ W, int64_t
             ucol[pcols[lane] - KC] = (int32_t)x;
         }
         if (zero_row) ucol[c0 - KC + lane] = 0;
+    }
+}
+
+constexpr int MAXPAN = 32;  // inner panels per outer block (NBO / smallest panel width)
+
+// End of an outer block with P pivots (rows [KR, KR+P)), whose inner panels
+// left their row gather maps in bmap_*: one wave per column outside the block
+// [KC, FAR).  Columns left of the block get the swaps; far columns get the
+// swaps, then the solve with the block's L11 (P x P, in LDS; lanes hold rows
+// i and i+64), and their U rows go to Ubig (rows 0..consumed-1 indexed by
+// block column, zero for block columns without a pivot).
+__global__ __launch_bounds__(1024) void k_block_finish(int32_t *__restrict__ W, int64_t ldw, int64_t n, int KR,
+                                                       int KC, int FAR, int P, int consumed, int npanels,
+                                                       const int32_t *__restrict__ bmap_dst,
+                                                       const int32_t *__restrict__ bmap_src,
+                                                       const PanelCtl *__restrict__ bctl,
+                                                       const int32_t *__restrict__ pivinv,
+                                                       const int32_t *__restrict__ pivcol,
+                                                       int32_t *__restrict__ Ubig, int64_t ldub, ModP mp)
+{
+    __shared__ uint32_t Ls[128][129];
+    __shared__ uint32_t invs[128];
+    __shared__ int ucolmap[128];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int idx = tid; idx < P * P; idx += 1024) {
+        const int i = idx / P, j = idx % P;
+        Ls[i][j] = j < i ? (uint32_t)W[(int64_t)(KR + i) + (int64_t)pivcol[KR + j] * ldw] : 0u;
+    }
+    if (tid < 128) ucolmap[tid] = -1;
+    __syncthreads();
+    if (tid < P) {
+        invs[tid] = (uint32_t)pivinv[KR + tid];
+        ucolmap[pivcol[KR + tid] - KC] = tid;
+    }
+    __syncthreads();
+    const int64_t nout = n - (FAR - KC);
+    const int um0 = lane < consumed ? ucolmap[lane] : -1;
+    const int um1 = lane + 64 < consumed ? ucolmap[lane + 64] : -1;
+    for (int64_t ci = (int64_t)blockIdx.x * 16 + wave; ci < nout; ci += (int64_t)gridDim.x * 16) {
+        int64_t c = ci < KC ? ci : ci + (FAR - KC);
+        int32_t *col = W + c * ldw;
+        for (int t = 0; t < npanels; ++t) {
+            const int nm = bctl[t].nmap;
+            const int32_t *md = bmap_dst + t * 64, *ms = bmap_src + t * 64;
+            uint32_t v = 0;
+            int d = 0;
+            if (lane < nm) { v = (uint32_t)col[ms[lane]]; d = md[lane]; }
+            if (lane < nm) col[d] = (int32_t)v;
+            __threadfence_block();
+        }
+        if (c < KC) continue;
+        uint32_t x0 = lane < P ? (uint32_t)col[KR + lane] : 0u;
+        uint32_t x1 = lane + 64 < P ? (uint32_t)col[KR + lane + 64] : 0u;
+        for (int j = 0; j < P; ++j) {
+            uint32_t xj;
+            if (j < 64) {
+                if (lane == j) x0 = mulmod(x0, invs[j], mp);
+                xj = (uint32_t)__shfl((int)x0, j);
+                if (lane > j && lane < P) x0 = submod(x0, mulmod(Ls[lane][j], xj, mp), mp);
+            } else {
+                if (lane == j - 64) x1 = mulmod(x1, invs[j], mp);
+                xj = (uint32_t)__shfl((int)x1, j - 64);
+            }
+            if (lane + 64 > j && lane + 64 < P) x1 = submod(x1, mulmod(Ls[lane + 64][j], xj, mp), mp);
+        }
+        if (lane < P) col[KR + lane] = (int32_t)x0;
+        if (lane + 64 < P) col[KR + lane + 64] = (int32_t)x1;
+        // Ubig rows 0..consumed-1 of this column: pivot i's value or zero
+        int32_t *ucol = Ubig + (c - KC) * ldub;
+        const int a0 = um0 < 0 ? 0 : um0, a1 = um1 < 0 ? 0 : um1;
+        const uint32_t y00 = (uint32_t)__shfl((int)x0, a0 & 63), y01 = (uint32_t)__shfl((int)x1, a0 & 63);
+        const uint32_t y10 = (uint32_t)__shfl((int)x0, a1 & 63), y11 = (uint32_t)__shfl((int)x1, a1 & 63);
+        if (lane < consumed) ucol[lane] = um0 < 0 ? 0 : (int32_t)(a0 < 64 ? y00 : y01);
+        if (lane + 64 < consumed) ucol[lane + 64] = um1 < 0 ? 0 : (int32_t)(a1 < 64 ? y10 : y11);
     }
 }
 
@@ -427,14 +500,16 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
     const int64_t ldw = rup(m, 64);
     int32_t *W = (int32_t *)ws_get(ctx, WS_WORK, sizeof(int32_t) * (size_t)ldw * (size_t)n, &st);
     if (!W) return st;
-    // pivot arrays + panel control + maps
+    // pivot arrays, per-panel control blocks and row gather maps of one outer block
     const size_t npv = (size_t)mn + 64;
-    int32_t *piv = (int32_t *)ws_get(ctx, WS_PIV, sizeof(int32_t) * (4 * npv + 4 * 64) + sizeof(PanelCtl) + 64, &st);
+    const size_t piv_bytes = sizeof(int32_t) * (4 * npv + 64 + 2 * MAXPAN * 64) + sizeof(PanelCtl) * MAXPAN + 64;
+    int32_t *piv = (int32_t *)ws_get(ctx, WS_PIV, piv_bytes, &st);
     if (!piv) return st;
     int32_t *ipiv = piv, *pivval = piv + npv, *pivinv = piv + 2 * npv, *pivcol = piv + 3 * npv;
-    int32_t *colmap = piv + 4 * npv, *map_dst = colmap + 64, *map_src = map_dst + 64;
-    int *flag = (int *)(map_src + 128);
-    PanelCtl *ctl = (PanelCtl *)(flag + 4);
+    int32_t *colmap = piv + 4 * npv;
+    int32_t *bmap_dst = colmap + 64, *bmap_src = bmap_dst + MAXPAN * 64;
+    PanelCtl *bctl = (PanelCtl *)(bmap_src + MAXPAN * 64);
+    int *flag = (int *)(bctl + MAXPAN);
     int32_t *Ubig = (int32_t *)ws_get(ctx, WS_UBUF, sizeof(int32_t) * (size_t)NBO * (size_t)(n + 1), &st);
     if (!Ubig) return st;
     PanelCtl *hctl = (PanelCtl *)pinned_get(ctx, 64, &st);
@@ -457,36 +532,36 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
     };
     const int big_form = form == MODP_FORM_AUTO ? choose_form(m, n, NBO, p) : form;
 
-    std::vector<int32_t> hpiv;  // ipiv gathered at the end
     std::vector<int64_t> col_stack;
     int64_t kr = 0, kc = 0, last = n - 1;
     bool done = false;
     while (!done && kr < m && kc < n) {
-        // ---- one outer block [KC, KC + nbo) ----
+        // ---- one outer block of columns [KC, FAR) ----
         const int64_t KR = kr, KC = kc;
         const int64_t colend = ple ? n : last + 1;  // columns usable as pivot columns
         int64_t nbo = colend - KC;
         if (nbo > NBO) nbo = NBO;
-        if (nbo <= 0) {
-            // pluq: only moved zero columns remain -> tail
-            nbo = 0;
-        }
+        if (nbo < 0) nbo = 0;  // pluq: only moved zero columns remain
         const int64_t FAR = KC + nbo;
-        int64_t consumed_total = 0;
         bool stop_zero = false;
+        int npanels = 0;
+        int64_t P = 0;
         while (nbo > 0 && kc < FAR && kr < m) {
             const int64_t cj = kc, rj = kr;
             int64_t w = FAR - cj;
             if (w > NBW) w = NBW;
             if (!ple && w > m - rj) w = m - rj;
+            int32_t *mdst = bmap_dst + npanels * 64, *msrc = bmap_src + npanels * 64;
+            PanelCtl *pctl = bctl + npanels;
             switch (RPT) {
-            case 1: MODP_TRY((launch_panel<1, 32>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, map_dst, map_src, ctl))); break;
-            case 2: MODP_TRY((launch_panel<2, 32>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, map_dst, map_src, ctl))); break;
-            case 4: MODP_TRY((launch_panel<4, 16>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, map_dst, map_src, ctl))); break;
-            case 8: MODP_TRY((launch_panel<8, 8>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, map_dst, map_src, ctl))); break;
-            default: MODP_TRY((launch_panel<16, 4>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, map_dst, map_src, ctl))); break;
+            case 1: MODP_TRY((launch_panel<1, 32>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, mdst, msrc, pctl))); break;
+            case 2: MODP_TRY((launch_panel<2, 32>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, mdst, msrc, pctl))); break;
+            case 4: MODP_TRY((launch_panel<4, 16>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, mdst, msrc, pctl))); break;
+            case 8: MODP_TRY((launch_panel<8, 8>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, mdst, msrc, pctl))); break;
+            default: MODP_TRY((launch_panel<16, 4>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, mdst, msrc, pctl))); break;
             }
-            HIP_TRY(ctx, hipMemcpyAsync(hctl, ctl, sizeof(PanelCtl), hipMemcpyDeviceToHost, ctx->stream));
+            ++npanels;
+            HIP_TRY(ctx, hipMemcpyAsync(hctl, pctl, sizeof(PanelCtl), hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
             const PanelCtl pc = *hctl;
             if (pc.status == 3)
@@ -494,37 +569,42 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
                                  (long long)p);
             const int64_t npiv = pc.npiv;
             if (npiv > 0) {
-                // swaps everywhere outside the panel; solve right of it
-                const int64_t nout = n - w;
-                if (nout > 0) {
+                // the rest of the outer block: swaps; solve right of the panel
+                const int64_t nblk = (FAR - KC) - w;
+                if (nblk > 0) {
                     TimedLaunch tl(ctx, "elim_finish");
-                    unsigned g = grid_for(nout * 64, 4096);
-                    k_finish<<<g, 256, 0, ctx->stream>>>(W, ldw, n, (int)rj, (int)cj, (int)w, (int)KC, ctl, pivinv,
-                                                         pivcol, colmap, map_dst, map_src, Ubig, NBO, cj + w, mp);
+                    k_finish<<<grid_for(nblk * 64, 4096), 256, 0, ctx->stream>>>(
+                        W, ldw, KC, FAR, (int)rj, (int)cj, (int)w, (int)KC, pctl, pivinv, pivcol, colmap, mdst, msrc,
+                        Ubig, NBO, mp);
                     HIP_TRY(ctx, hipGetLastError());
                 }
+                // G1: rest of the outer block, every row below the pivots
                 const int64_t r1 = rj + npiv;
-                // G1: rest of the outer block, all rows below
                 MODP_TRY(gemm_sub(r1, m - r1, cj + w, FAR - (cj + w), cj, w, Ubig + (cj - KC) + (cj + w - KC) * NBO,
-                                  MODP_FORM_PLAIN_F64));
-                // G2: far columns, rows still inside this outer block
-                int64_t rend = KR + nbo;
-                if (rend > m) rend = m;
-                MODP_TRY(gemm_sub(r1, rend - r1, FAR, n - FAR, cj, w, Ubig + (cj - KC) + (FAR - KC) * NBO,
                                   MODP_FORM_PLAIN_F64));
             }
             kr += npiv;
             kc += ple ? pc.ncols : npiv;
-            consumed_total = kc - KC;
+            P += npiv;
             if (pc.status == 1) { stop_zero = true; break; }
             if (pc.status == 2) break;
         }
-        // big trailing update for rows below the outer block
-        {
-            int64_t rbeg = KR + nbo;
-            if (rbeg < m && nbo > 0 && consumed_total > 0)
-                MODP_TRY(gemm_sub(rbeg, m - rbeg, FAR, n - FAR, KC, consumed_total, Ubig + (FAR - KC) * NBO,
-                                  big_form));
+        const int64_t consumed = kc - KC;
+        if (P > 0) {
+            // columns outside the block: the block's swaps; far columns: solve
+            // with the block's L11, then the trailing GEMM with K = consumed
+            const int64_t nout = n - nbo;
+            if (nout > 0) {
+                TimedLaunch tl(ctx, "elim_block_finish");
+                int64_t g = (nout + 15) / 16;
+                if (g > 2048) g = 2048;
+                k_block_finish<<<(unsigned)g, 1024, 0, ctx->stream>>>(W, ldw, n, (int)KR, (int)KC, (int)FAR, (int)P,
+                                                                      (int)consumed, npanels, bmap_dst, bmap_src,
+                                                                      bctl, pivinv, pivcol, Ubig, NBO, mp);
+                HIP_TRY(ctx, hipGetLastError());
+            }
+            const int64_t rbeg = KR + P;
+            MODP_TRY(gemm_sub(rbeg, m - rbeg, FAR, n - FAR, KC, consumed, Ubig + (FAR - KC) * NBO, big_form));
         }
         if (!ple && kr < m && kc < n && (stop_zero || nbo == 0 || kc > last)) {
             // pluq zero column at kc (rows >= kr zero): whole block zero -> tail,
