@@ -203,6 +203,7 @@ int modp_ctx_destroy(modp_ctx *ctx)
     for (int i = 0; i < 8; ++i)
         if (ctx->ws[i]) hipFree(ctx->ws[i]);
     if (ctx->pinned) hipHostFree(ctx->pinned);
+    if (ctx->invtab) hipFree(ctx->invtab);
     for (auto &t : ctx->timings) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     for (auto e : ctx->event_pool) hipEventDestroy(e);
     for (auto e : ctx->slots)

@@ -141,6 +141,10 @@ struct modp_ctx {
     std::vector<KernelTiming> timings;
     std::vector<hipEvent_t> event_pool;
     hipEvent_t slots[16] = {nullptr};
+    // table of modular inverses for the current elimination modulus
+    uint32_t *invtab = nullptr;
+    int64_t invtab_p = 0;
+    size_t invtab_cap = 0;
 };
 
 // workspace slots
@@ -198,6 +202,9 @@ struct GemmArgs {
     void *C; int64_t ldc; int kc;
     int epi;
     ModP mp;
+    // optional device flag: when nonzero at launch time the kernels return
+    // without touching memory (the elimination's optimistic pipeline)
+    const int *skip = nullptr;
 };
 
 int gemm_plain_f64(modp_ctx *ctx, const GemmArgs &g);

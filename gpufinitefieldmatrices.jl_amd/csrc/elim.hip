@@ -26,9 +26,25 @@
 //                rows) and of the rows of the far columns that are still in
 //                this outer block.
 //   after the outer block: one GEMM with K = NBO for the far trailing block.
+#include <stdlib.h>
+
+#include <type_traits>
+
 #include "common.h"
 
 namespace {
+
+// compile-time loop: f(integral_constant<int, J>) for J = B .. E-1
+template <int B, int E> struct Unroll {
+    template <class F> __device__ __forceinline__ static void run(F &f)
+    {
+        f(std::integral_constant<int, B>{});
+        Unroll<B + 1, E>::run(f);
+    }
+};
+template <int E> struct Unroll<E, E> {
+    template <class F> __device__ __forceinline__ static void run(F &) {}
+};
 
 constexpr int NBO = 128;  // outer block width (K of the trailing GEMM)
 
@@ -40,71 +56,213 @@ struct PanelCtl {
     int nmap;    // entries of the row gather map
 };
 
-template <int RPT, int NBW>
-__global__ __launch_bounds__(1024) void k_panel(int32_t *__restrict__ W, int64_t ldw, int m, int kr0, int c0,
-                                                int w, ModP mp, int ple, int32_t *__restrict__ ipiv,
-                                                int32_t *__restrict__ pivval, int32_t *__restrict__ pivinv,
-                                                int32_t *__restrict__ pivcol, int32_t *__restrict__ colmap,
-                                                int32_t *__restrict__ map_dst, int32_t *__restrict__ map_src,
-                                                PanelCtl *__restrict__ ctl)
+constexpr int PT = 512;  // threads of the panel workgroup (8 waves, 256-VGPR budget)
+
+// a^{-1} mod p for p < 2^24: extended Euclid whose quotients come from a
+// float reciprocal (exact operands below 2^24, estimate corrected to the
+// exact quotient); larger p use the double-division variant.
+__device__ __forceinline__ uint32_t modinv_fast(uint32_t a, const ModP &mp)
 {
-    __shared__ uint32_t s_val[2][16];
-    __shared__ int s_idx[2][16];
-    __shared__ uint32_t s_prow[NBW], s_crow[NBW], s_nrow[NBW];
-    __shared__ uint32_t s_inv;
-    __shared__ int s_ipiv[NBW];
+    if (mp.p >= (1 << 24)) return modinv(a, mp);
+    int32_t inv = 0, ninv = 1, rem = (int32_t)mp.p32, nrem = (int32_t)a;
+    while (nrem != 0) {
+        int32_t q = (int32_t)((float)rem * __builtin_amdgcn_rcpf((float)nrem));
+        int32_t r = rem - q * nrem;
+        while (r < 0) { --q; r += nrem; }
+        while (r >= nrem) { ++q; r -= nrem; }
+        int32_t t = inv - q * ninv; inv = ninv; ninv = t;
+        rem = nrem; nrem = r;
+    }
+    if (rem != 1) return 0;
+    if (inv < 0) inv += (int32_t)mp.p32;
+    return (uint32_t)inv;
+}
+
+template <bool SMALL> __device__ __forceinline__ uint32_t mulmod_t(uint32_t a, uint32_t b, const ModP &mp)
+{
+    if (SMALL) {
+        // p < 2^16: operands and quotient fit 24 bits, so the full-rate
+        // 24-bit multiplier gives the exact 32-bit products
+        const uint32_t x = __umul24(a, b);
+        const uint32_t q = (uint32_t)((float)x * mp.pinvf);
+        int32_t r = (int32_t)(x - __umul24(q, mp.p32));
+        r += r < 0 ? (int32_t)mp.p32 : 0;
+        r -= r >= (int32_t)mp.p32 ? (int32_t)mp.p32 : 0;
+        return (uint32_t)r;
+    }
+    return (uint32_t)dmod((double)a * (double)b, mp);
+}
+
+// Pivot-search keys: larger key = larger residue, then lower position.
+template <bool SMALL> struct PivKey;
+template <> struct PivKey<true> {
+    typedef uint32_t T;  // residue < 2^16 in bits 30..15, 0x7fff - position below
+    static __device__ __forceinline__ T make(uint32_t v, int pos) { return (v << 15) | (uint32_t)(0x7fff - pos); }
+    static __device__ __forceinline__ uint32_t val(T k) { return k >> 15; }
+    static __device__ __forceinline__ int pos(T k) { return 0x7fff - (int)(k & 0x7fffu); }
+};
+template <> struct PivKey<false> {
+    typedef uint64_t T;
+    static __device__ __forceinline__ T make(uint32_t v, int pos) { return ((uint64_t)v << 32) | (uint32_t)(0x7fffffff - pos); }
+    static __device__ __forceinline__ uint32_t val(T k) { return (uint32_t)(k >> 32); }
+    static __device__ __forceinline__ int pos(T k) { return 0x7fffffff - (int)(uint32_t)k; }
+};
+
+template <int CTRL> __device__ __forceinline__ uint32_t dpp32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL> __device__ __forceinline__ uint64_t dpp64(uint64_t v)
+{
+    return ((uint64_t)dpp32<CTRL>((uint32_t)(v >> 32)) << 32) | dpp32<CTRL>((uint32_t)v);
+}
+template <typename T> __device__ __forceinline__ T kmax(T a, T b) { return a > b ? a : b; }
+
+// wave64 maximum: DPP within each row of 16 lanes (quad swaps, half-row and
+// row mirrors), then the four row results through readlane
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+    v = kmax(v, dpp32<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = kmax(v, dpp32<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = kmax(v, dpp32<0x141>(v));  // row_half_mirror
+    v = kmax(v, dpp32<0x140>(v));  // row_mirror
+    const uint32_t r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const uint32_t r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return kmax(kmax(r0, r1), kmax(r2, r3));
+}
+__device__ __forceinline__ uint64_t wave_max(uint64_t v)
+{
+    v = kmax(v, dpp64<0xB1>(v));
+    v = kmax(v, dpp64<0x4E>(v));
+    v = kmax(v, dpp64<0x141>(v));
+    v = kmax(v, dpp64<0x140>(v));
+    uint64_t r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        r[i] = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), 16 * i) << 32) |
+               __builtin_amdgcn_readlane((uint32_t)v, 16 * i);
+    return kmax(kmax(r[0], r[1]), kmax(r[2], r[3]));
+}
+
+// Panel factorisation: one 512-thread workgroup, the panel W[kr0:m, c0:c0+w]
+// in registers (thread t owns local rows t + 512*q) as exact integers in
+// FP64.  Each rank-1 update is one FMA: for p < 2^16 every product is below
+// 2^32, so up to 2^21 updates stay exact below 2^53 and only the pivot
+// column and the pivot row are reduced (lazy reduction); larger p reduce
+// after every update.
+//
+// One barrier per column: every wave finds its best candidate (largest
+// residue, then lowest position) with DPP, and the lane holding it publishes
+// that row, its index and its inverse in LDS before the barrier (double
+// buffered by column parity); after the barrier all threads pick the winner
+// among the waves and already have its row.  Rows never move: each row
+// carries its current position (the reference's swap of positions pc and
+// Ip), a row chosen as pivot is frozen with its normalised U row kept in
+// LDS, and at the end every row is written to its final position.
+template <int RPT, int NBW, bool SMALL>
+__global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t ldw, int m, int kr0, int c0,
+                                              int w, ModP mp, int ple, const uint32_t *__restrict__ invtab,
+                                              int32_t *__restrict__ ipiv, int32_t *__restrict__ pivval,
+                                              int32_t *__restrict__ pivinv, int32_t *__restrict__ pivcol,
+                                              int32_t *__restrict__ colmap, int32_t *__restrict__ map_dst,
+                                              int32_t *__restrict__ map_src, PanelCtl *__restrict__ ctl,
+                                              int *__restrict__ abort)
+{
+    constexpr int NW = PT / 64;
+    if (abort && *abort) {
+        // an earlier panel of this block stopped: nothing to do
+        if (threadIdx.x == 0) {
+            ctl->npiv = 0;
+            ctl->status = 4;
+            ctl->ncols = 0;
+            ctl->nmap = 0;
+        }
+        return;
+    }
+    typedef PivKey<SMALL> K;
+    typedef typename K::T key_t;
+    __shared__ key_t s_key[2][NW];
+    __shared__ uint32_t s_crow_val[2][NW][NBW];  // candidate rows (residues)
+    __shared__ uint32_t s_cinv[2][NW];
+    __shared__ int s_cidx[2][NW];                // candidate local row index
+    __shared__ uint32_t s_U[NBW][NBW];
+    __shared__ int s_pcol[NBW];
+    __shared__ int s_cnt;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rows = m - kr0;
-    uint32_t v[RPT][NBW];
+    double v[RPT][NBW];
+    int pos[RPT];
+    uint32_t pivmask = 0;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-        const int i = tid + 1024 * q;
+        const int i = tid + PT * q;
+        pos[q] = i;
 #pragma unroll
         for (int j = 0; j < NBW; ++j)
-            v[q][j] = (i < rows && j < w) ? (uint32_t)W[(int64_t)(kr0 + i) + (int64_t)(c0 + j) * ldw] : 0u;
+            v[q][j] = (i < rows && j < w) ? (double)W[(int64_t)(kr0 + i) + (int64_t)(c0 + j) * ldw] : 0.0;
     }
+    if (tid == 0) s_cnt = 0;
 
     int pc = 0, status = 0, consumed = w, par = 0;
     bool active = true;
-    // fully unrolled over the panel's columns so that v[][] stays in registers
-#pragma unroll
-    for (int jc = 0; jc < NBW; ++jc) {
-        if (!active || jc >= w) continue;
+    // one step per panel column, unrolled at compile time so that v[][jc]
+    // is a register
+    auto step = [&](auto JC) __attribute__((always_inline)) {
+        constexpr int jc = decltype(JC)::value;
+        if (!active || jc >= w) return;
         if (pc >= rows) {
             status = 2;
             consumed = jc;
             active = false;
-            continue;
+            return;
         }
-        // candidate of this thread: largest value, lowest row
-        uint32_t bv = 0;
-        int bi = 0x7fffffff;
+        // this thread's best row among those not yet pivots
+        key_t mine = 0;
+        int mq = 0;
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            const int i = tid + 1024 * q;
-            if (i >= pc && i < rows) {
-                uint32_t x = v[q][jc];
-                if (x > bv) { bv = x; bi = i; }
+            if (SMALL) v[q][jc] = dmod(v[q][jc], mp);
+            const int i = tid + PT * q;
+            if (i < rows && !((pivmask >> q) & 1u)) {
+                const key_t k = K::make((uint32_t)v[q][jc], pos[q]);
+                if (k > mine) { mine = k; mq = q; }
             }
         }
+        const key_t wbest = wave_max(mine);
+        if (mine == wbest && K::val(wbest) != 0) {
+            // this lane holds the wave's candidate: publish row, index, inverse
+            const uint32_t V = K::val(wbest);
+            s_cinv[par][wave] = invtab ? invtab[V] : modinv_fast(V, mp);
+            s_cidx[par][wave] = tid + PT * mq;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            uint32_t ov = __shfl_xor(bv, off);
-            int oi = __shfl_xor(bi, off);
-            if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+            for (int j = 0; j < NBW; ++j)
+                if (j > jc) {
+                    // register select of row mq (an empty asm keeps the
+                    // compiler from turning the chain into an indexed load,
+                    // which would move v[][] to scratch memory)
+                    double x = v[0][j];
+#pragma unroll
+                    for (int q = 1; q < RPT; ++q) {
+                        x = q == mq ? v[q][j] : x;
+                        asm volatile("" : "+v"(x));
+                    }
+                    s_crow_val[par][wave][j] = (uint32_t)dmod(x, mp);
+                }
         }
-        if (lane == 0) { s_val[par][wave] = bv; s_idx[par][wave] = bi; }
+        if (lane == 0) s_key[par][wave] = wbest;
         __syncthreads();
-        uint32_t V = s_val[par][0];
-        int I = s_idx[par][0];
+        key_t best = s_key[par][0];
+        int bw = 0;
 #pragma unroll
-        for (int x = 1; x < 16; ++x) {
-            uint32_t ov = s_val[par][x];
-            int oi = s_idx[par][x];
-            if (ov > V || (ov == V && oi < I)) { V = ov; I = oi; }
+        for (int x = 1; x < NW; ++x) {
+            const key_t k = s_key[par][x];
+            if (k > best) { best = k; bw = x; }
         }
+        const int cp = par;
         par ^= 1;
+        const uint32_t V = K::val(best);
+        const int Ip = K::pos(best);
         if (V == 0) {
             if (ple) {
                 if (tid == 0) colmap[jc] = -1;
@@ -113,123 +271,94 @@ __global__ __launch_bounds__(1024) void k_panel(int32_t *__restrict__ W, int64_t
                 consumed = jc;
                 active = false;
             }
-            continue;
+            return;
         }
-        const int tI = I & 1023, qI = I >> 10, tP = pc & 1023, qP = pc >> 10;
-        if (tid == tI) {
-            const uint32_t inv = modinv(V, mp);
-            s_inv = inv;
-#pragma unroll
-            for (int q = 0; q < RPT; ++q)
-                if (q == qI) {
-#pragma unroll
-                    for (int j = 0; j < NBW; ++j) {
-                        s_prow[j] = v[q][j];
-                        if (j > jc) s_nrow[j] = mulmod(v[q][j], inv, mp);
-                    }
-                }
-        }
-        if (tid == tP) {
-#pragma unroll
-            for (int q = 0; q < RPT; ++q)
-                if (q == qP) {
-#pragma unroll
-                    for (int j = 0; j < NBW; ++j) s_crow[j] = v[q][j];
-                }
-        }
-        __syncthreads();
-        const uint32_t inv = s_inv;
+        const uint32_t inv = s_cinv[cp][bw];
         if (inv == 0) {
             status = 3;
             consumed = jc;
             active = false;
-            continue;
+            return;
         }
-        if (tid == tI) {
+        const int wrow = s_cidx[cp][bw];
+        // positions: the winner moves to pc, the row at pc moves to Ip
 #pragma unroll
-            for (int q = 0; q < RPT; ++q)
-                if (q == qI) {
-#pragma unroll
-                    for (int j = 0; j < NBW; ++j) v[q][j] = s_crow[j];
-                }
+        for (int q = 0; q < RPT; ++q) {
+            const int i = tid + PT * q;
+            if (i == wrow) { pos[q] = pc; pivmask |= 1u << q; }
+            else if (pos[q] == pc) pos[q] = Ip;
         }
-        if (tid == tP) {
+        // normalised pivot row, then the rank-1 update of the rows that are
+        // not pivots (frozen and out-of-range rows get multiplier 0)
+        double u[NBW];
 #pragma unroll
-            for (int q = 0; q < RPT; ++q)
-                if (q == qP) {
-#pragma unroll
-                    for (int j = 0; j < NBW; ++j) v[q][j] = j < jc ? s_prow[j] : (j == jc ? 1u : s_nrow[j]);
-                }
+        for (int j = 0; j < NBW; ++j) {
+            const uint32_t uj = j > jc ? mulmod_t<SMALL>(s_crow_val[cp][bw][j], inv, mp) : 0u;
+            if (tid == 0 && j > jc) s_U[pc][j] = uj;
+            u[j] = (double)uj;
         }
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            const int i = tid + 1024 * q;
-            if (i > pc && i < rows) {
-                const uint32_t l = v[q][jc];
-                if (l) {
+            const int i = tid + PT * q;
+            const double l = (i < rows && !((pivmask >> q) & 1u)) ? v[q][jc] : 0.0;
 #pragma unroll
-                    for (int j = 0; j < NBW; ++j)
-                        if (j > jc) v[q][j] = submod(v[q][j], mulmod(l, s_nrow[j], mp), mp);
+            for (int j = 0; j < NBW; ++j)
+                if (j > jc) {
+                    v[q][j] = fma(-l, u[j], v[q][j]);
+                    if (!SMALL) v[q][j] = dmod(v[q][j], mp);
                 }
-            }
         }
         if (tid == 0) {
-            ipiv[kr0 + pc] = kr0 + I;
+            ipiv[kr0 + pc] = kr0 + Ip;
             pivval[kr0 + pc] = (int32_t)V;
             pivinv[kr0 + pc] = (int32_t)inv;
             pivcol[kr0 + pc] = c0 + jc;
             colmap[jc] = pc;
-            s_ipiv[pc] = kr0 + I;
+            s_pcol[pc] = jc;
         }
         ++pc;
-    }
+    };
+    Unroll<0, NBW>::run(step);
+    __syncthreads();
 
+    // every row to its final position; pivot rows take 1 at their pivot
+    // column and the normalised U row after it.  Row gather map: positions
+    // 0..pc-1 first, in order, then every other row that moved.
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-        const int i = tid + 1024 * q;
+        const int i = tid + PT * q;
         if (i < rows) {
+            const int dst = kr0 + pos[q];
+            if ((pivmask >> q) & 1u) {
+                const int k = pos[q], jk = s_pcol[k];
 #pragma unroll
-            for (int j = 0; j < NBW; ++j)
-                if (j < w) W[(int64_t)(kr0 + i) + (int64_t)(c0 + j) * ldw] = (int32_t)v[q][j];
+                for (int j = 0; j < NBW; ++j)
+                    if (j < w)
+                        W[dst + (int64_t)(c0 + j) * ldw] =
+                            (int32_t)(j < jk ? (uint32_t)dmod(v[q][j], mp) : (j == jk ? 1u : s_U[k][j]));
+            } else {
+#pragma unroll
+                for (int j = 0; j < NBW; ++j)
+                    if (j < w) W[dst + (int64_t)(c0 + j) * ldw] = (int32_t)(uint32_t)dmod(v[q][j], mp);
+            }
+            if (pos[q] < pc) {
+                map_dst[pos[q]] = dst;
+                map_src[pos[q]] = kr0 + i;
+            } else if (pos[q] != i) {
+                const int e = pc + atomicAdd(&s_cnt, 1);
+                map_dst[e] = dst;
+                map_src[e] = kr0 + i;
+            }
         }
     }
-
+    __syncthreads();
     if (tid == 0) {
         for (int j = consumed; j < w; ++j) colmap[j] = -1;
-        // gather map of the row swaps: rows kr0..kr0+pc-1 first, in order,
-        // then every other row whose content changed
-        int rowv[2 * NBW], orig[2 * NBW], cnt = 0;
-        for (int i = 0; i < pc; ++i) {
-            const int a = kr0 + i, b = s_ipiv[i];
-            if (a == b) continue;
-            int ia = -1, ib = -1;
-            for (int e = 0; e < cnt; ++e) {
-                if (rowv[e] == a) ia = e;
-                if (rowv[e] == b) ib = e;
-            }
-            if (ia < 0) { rowv[cnt] = a; orig[cnt] = a; ia = cnt++; }
-            if (ib < 0) { rowv[cnt] = b; orig[cnt] = b; ib = cnt++; }
-            int t = orig[ia]; orig[ia] = orig[ib]; orig[ib] = t;
-        }
-        int n = 0;
-        for (int i = 0; i < pc; ++i) {
-            int src = kr0 + i;
-            for (int e = 0; e < cnt; ++e)
-                if (rowv[e] == kr0 + i) src = orig[e];
-            map_dst[n] = kr0 + i;
-            map_src[n] = src;
-            ++n;
-        }
-        for (int e = 0; e < cnt; ++e)
-            if (rowv[e] >= kr0 + pc && orig[e] != rowv[e]) {
-                map_dst[n] = rowv[e];
-                map_src[n] = orig[e];
-                ++n;
-            }
         ctl->npiv = pc;
         ctl->status = status;
         ctl->ncols = consumed;
-        ctl->nmap = n;
+        ctl->nmap = pc + s_cnt;
+        if (abort && status != 0) *abort = 1;
     }
 }
 
@@ -245,8 +374,9 @@ __global__ __launch_bounds__(256) void k_finish(int32_t *__restrict__ W, int64_t
                                                 const int32_t *__restrict__ colmap,
                                                 const int32_t *__restrict__ map_dst,
                                                 const int32_t *__restrict__ map_src, int32_t *__restrict__ Ubig,
-                                                int64_t ldub, ModP mp)
+                                                int64_t ldub, ModP mp, const int *__restrict__ skip)
 {
+    if (skip && *skip) return;
     __shared__ uint32_t Ls[64][65];
     __shared__ uint32_t invs[64];
     __shared__ int pcols[64];
@@ -305,8 +435,10 @@ __global__ __launch_bounds__(1024) void k_block_finish(int32_t *__restrict__ W, 
                                                        const PanelCtl *__restrict__ bctl,
                                                        const int32_t *__restrict__ pivinv,
                                                        const int32_t *__restrict__ pivcol,
-                                                       int32_t *__restrict__ Ubig, int64_t ldub, ModP mp)
+                                                       int32_t *__restrict__ Ubig, int64_t ldub, ModP mp,
+                                                       const int *__restrict__ skip)
 {
+    if (skip && *skip) return;
     __shared__ uint32_t Ls[128][129];
     __shared__ uint32_t invs[128];
     __shared__ int ucolmap[128];
@@ -454,12 +586,17 @@ __global__ __launch_bounds__(256) void k_trsm_upper_unit(int32_t *__restrict__ W
 
 template <int RPT, int NBW>
 int launch_panel(modp_ctx *ctx, int32_t *W, int64_t ldw, int m, int kr0, int c0, int w, const ModP &mp, int ple,
-                 int32_t *ipiv, int32_t *pivval, int32_t *pivinv, int32_t *pivcol, int32_t *colmap,
-                 int32_t *map_dst, int32_t *map_src, PanelCtl *ctl)
+                 const uint32_t *invtab, int32_t *ipiv, int32_t *pivval, int32_t *pivinv, int32_t *pivcol,
+                 int32_t *colmap, int32_t *map_dst, int32_t *map_src, PanelCtl *ctl, int *abort)
 {
     TimedLaunch tl(ctx, "elim_panel");
-    k_panel<RPT, NBW><<<1, 1024, 0, ctx->stream>>>(W, ldw, m, kr0, c0, w, mp, ple, ipiv, pivval, pivinv, pivcol,
-                                                   colmap, map_dst, map_src, ctl);
+    const size_t shm = 0;
+    if (mp.small)
+        k_panel<RPT, NBW, true><<<1, PT, shm, ctx->stream>>>(W, ldw, m, kr0, c0, w, mp, ple, invtab, ipiv, pivval,
+                                                            pivinv, pivcol, colmap, map_dst, map_src, ctl, abort);
+    else
+        k_panel<RPT, NBW, false><<<1, PT, shm, ctx->stream>>>(W, ldw, m, kr0, c0, w, mp, ple, invtab, ipiv, pivval,
+                                                             pivinv, pivcol, colmap, map_dst, map_src, ctl, abort);
     return hip_check(ctx, hipGetLastError(), "k_panel launch");
 }
 
@@ -472,6 +609,37 @@ unsigned grid_for(int64_t total, int64_t cap = 16384)
 }
 
 int64_t rup(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+__global__ void k_inv_table(uint32_t *__restrict__ t, ModP mp)
+{
+    for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < mp.p; a += (int64_t)gridDim.x * blockDim.x)
+        t[a] = a == 0 ? 0u : modinv((uint32_t)a, mp);
+}
+
+// Inverses of every residue for p <= 2^22 (at most 16 MB), built once per
+// modulus and kept in the context: the panel's pivot inverse becomes one load.
+const uint32_t *inverse_table(modp_ctx *ctx, const ModP &mp, int *st)
+{
+    if (mp.p > ((int64_t)1 << 22)) return nullptr;
+    if (ctx->invtab && ctx->invtab_p == mp.p) return ctx->invtab;
+    const size_t need = sizeof(uint32_t) * (size_t)mp.p;
+    if (ctx->invtab_cap < need) {
+        if (ctx->invtab) {
+            hipStreamSynchronize(ctx->stream);
+            hipFree(ctx->invtab);
+            ctx->invtab = nullptr;
+            ctx->invtab_cap = 0;
+        }
+        hipError_t e = hipMalloc(&ctx->invtab, need);
+        if (e != hipSuccess) { *st = hip_check(ctx, e, "inverse table"); return nullptr; }
+        ctx->invtab_cap = need;
+    }
+    k_inv_table<<<grid_for(mp.p, 4096), 256, 0, ctx->stream>>>(ctx->invtab, mp);
+    if (hipGetLastError() != hipSuccess) { *st = MODP_ERR_DEVICE; return nullptr; }
+    ctx->invtab_p = mp.p;
+    return ctx->invtab;
+}
+
 
 }  // namespace
 
@@ -493,8 +661,16 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
 
     // panel shape: rows per thread x panel width <= 64 registers
     int RPT = 1;
-    while ((int64_t)RPT * 1024 < m) RPT *= 2;
-    const int NBW = RPT <= 2 ? 32 : 64 / RPT;
+    while ((int64_t)RPT * PT < m) RPT *= 2;
+    // panel width per rows-per-thread, measured at 2048 and 4096 rows
+    int NBW = RPT == 1 ? 32 : (RPT == 2 ? 16 : (RPT <= 8 ? 8 : 2));
+    if (const char *e = getenv("MODP_PANEL_W")) {
+        // tuning override among the instantiated panel widths
+        const int want = atoi(e);
+        if ((RPT <= 2 && (want == 16 || want == 32)) || (RPT == 4 && (want == 8 || want == 16)) ||
+            (RPT == 8 && (want == 4 || want == 8)))
+            NBW = want;
+    }
 
     int st = MODP_OK;
     const int64_t ldw = rup(m, 64);
@@ -512,13 +688,16 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
     int *flag = (int *)(bctl + MAXPAN);
     int32_t *Ubig = (int32_t *)ws_get(ctx, WS_UBUF, sizeof(int32_t) * (size_t)NBO * (size_t)(n + 1), &st);
     if (!Ubig) return st;
-    PanelCtl *hctl = (PanelCtl *)pinned_get(ctx, 64, &st);
+    PanelCtl *hctl = (PanelCtl *)pinned_get(ctx, sizeof(PanelCtl) * (MAXPAN + 1) + 64, &st);
     if (!hctl) return st;
+    int *abort_flag = flag + 1;
 
     MODP_TRY(util_convert(ctx, kind, m, n, A, lda, EK_I32, W, ldw, mp));
+    const uint32_t *invtab = inverse_table(ctx, mp, &st);
+    if (st != MODP_OK) return st;
 
     auto gemm_sub = [&](int64_t r0, int64_t rows, int64_t ca, int64_t cols, int64_t kcol, int64_t K,
-                        const int32_t *Ub, int use_form) -> int {
+                        const int32_t *Ub, int use_form, const int *skip) -> int {
         if (rows <= 0 || cols <= 0 || K <= 0) return MODP_OK;
         GemmArgs g;
         g.m = rows; g.n = cols; g.k = K;
@@ -527,10 +706,70 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
         g.C = W + r0 + ca * ldw; g.ldc = ldw; g.kc = EK_I32;
         g.epi = EPI_SUB;
         g.mp = mp;
+        g.skip = skip;
         if (use_form == MODP_FORM_LIMB_I8) return gemm_limb_i8(ctx, g);
         return gemm_plain_f64(ctx, g);
     };
     const int big_form = form == MODP_FORM_AUTO ? choose_form(m, n, NBO, p) : form;
+
+    // panel t of the current outer block at (rj, cj), width w
+    auto launch_panel_t = [&](int t, int64_t rj, int64_t cj, int64_t w, int *abort) -> int {
+        int32_t *mdst = bmap_dst + t * 64, *msrc = bmap_src + t * 64;
+        PanelCtl *pctl = bctl + t;
+        const int key = RPT * 100 + NBW;
+#define LAUNCH_PANEL(R_, W_) \
+    case R_ * 100 + W_: \
+        return launch_panel<R_, W_>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, invtab, ipiv, pivval, \
+                                    pivinv, pivcol, colmap, mdst, msrc, pctl, abort);
+        switch (key) {
+            LAUNCH_PANEL(1, 16)
+            LAUNCH_PANEL(1, 32)
+            LAUNCH_PANEL(2, 16)
+            LAUNCH_PANEL(2, 32)
+            LAUNCH_PANEL(4, 8)
+            LAUNCH_PANEL(4, 16)
+            LAUNCH_PANEL(8, 4)
+            LAUNCH_PANEL(8, 8)
+            LAUNCH_PANEL(16, 2)
+            LAUNCH_PANEL(32, 2)
+        default:
+            return set_error(ctx, MODP_ERR_ARG, "no panel kernel for %d rows/thread x %d columns", RPT, NBW);
+        }
+#undef LAUNCH_PANEL
+    };
+    // the rest of the outer block after panel t: swaps, solve right of the
+    // panel, then G1 = the rank-npiv update of those columns for every row below
+    auto finish_t = [&](int t, int64_t KC, int64_t FAR, int64_t rj, int64_t cj, int64_t w, int64_t npiv,
+                        const int *skip) -> int {
+        const int64_t nblk = (FAR - KC) - w;
+        if (nblk > 0) {
+            TimedLaunch tl(ctx, "elim_finish");
+            k_finish<<<grid_for(nblk * 64, 4096), 256, 0, ctx->stream>>>(
+                W, ldw, KC, FAR, (int)rj, (int)cj, (int)w, (int)KC, bctl + t, pivinv, pivcol, colmap,
+                bmap_dst + t * 64, bmap_src + t * 64, Ubig, NBO, mp, skip);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+        const int64_t r1 = rj + npiv;
+        return gemm_sub(r1, m - r1, cj + w, FAR - (cj + w), cj, w, Ubig + (cj - KC) + (cj + w - KC) * NBO,
+                        MODP_FORM_PLAIN_F64, skip);
+    };
+    // end of an outer block with P pivots over `consumed` columns
+    auto block_end = [&](int64_t KR, int64_t KC, int64_t FAR, int64_t P, int64_t consumed, int npanels,
+                         const int *skip) -> int {
+        if (P <= 0) return MODP_OK;
+        const int64_t nout = n - (FAR - KC);
+        if (nout > 0) {
+            TimedLaunch tl(ctx, "elim_block_finish");
+            int64_t g = (nout + 15) / 16;
+            if (g > 2048) g = 2048;
+            k_block_finish<<<(unsigned)g, 1024, 0, ctx->stream>>>(W, ldw, n, (int)KR, (int)KC, (int)FAR, (int)P,
+                                                                  (int)consumed, npanels, bmap_dst, bmap_src, bctl,
+                                                                  pivinv, pivcol, Ubig, NBO, mp, skip);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+        const int64_t rbeg = KR + P;
+        return gemm_sub(rbeg, m - rbeg, FAR, n - FAR, KC, consumed, Ubig + (FAR - KC) * NBO, big_form, skip);
+    };
 
     std::vector<int64_t> col_stack;
     int64_t kr = 0, kc = 0, last = n - 1;
@@ -544,67 +783,79 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
         if (nbo < 0) nbo = 0;  // pluq: only moved zero columns remain
         const int64_t FAR = KC + nbo;
         bool stop_zero = false;
-        int npanels = 0;
-        int64_t P = 0;
-        while (nbo > 0 && kc < FAR && kr < m) {
-            const int64_t cj = kc, rj = kr;
-            int64_t w = FAR - cj;
-            if (w > NBW) w = NBW;
-            if (!ple && w > m - rj) w = m - rj;
-            int32_t *mdst = bmap_dst + npanels * 64, *msrc = bmap_src + npanels * 64;
-            PanelCtl *pctl = bctl + npanels;
-            switch (RPT) {
-            case 1: MODP_TRY((launch_panel<1, 32>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, mdst, msrc, pctl))); break;
-            case 2: MODP_TRY((launch_panel<2, 32>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, mdst, msrc, pctl))); break;
-            case 4: MODP_TRY((launch_panel<4, 16>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, mdst, msrc, pctl))); break;
-            case 8: MODP_TRY((launch_panel<8, 8>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, mdst, msrc, pctl))); break;
-            default: MODP_TRY((launch_panel<16, 4>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, ipiv, pivval, pivinv, pivcol, colmap, mdst, msrc, pctl))); break;
+        if (!ple && nbo > 0) {
+            // pluq: every panel of the block is enqueued assuming it finds a
+            // pivot in each column; a panel that meets a zero column raises
+            // the abort flag, which turns the rest of the block into no-ops,
+            // and the block is completed from the host after one sync
+            HIP_TRY(ctx, hipMemsetAsync(abort_flag, 0, sizeof(int), ctx->stream));
+            std::vector<int64_t> pr, pcj, pw;
+            for (int64_t cj = KC; cj < FAR;) {
+                const int64_t rj = KR + (cj - KC);
+                if (rj >= m) break;
+                int64_t w = FAR - cj;
+                if (w > NBW) w = NBW;
+                if (w > m - rj) w = m - rj;
+                pr.push_back(rj); pcj.push_back(cj); pw.push_back(w);
+                cj += w;
             }
-            ++npanels;
-            HIP_TRY(ctx, hipMemcpyAsync(hctl, pctl, sizeof(PanelCtl), hipMemcpyDeviceToHost, ctx->stream));
+            const int np = (int)pr.size();
+            int64_t Pfull = 0;
+            for (int t = 0; t < np; ++t) {
+                MODP_TRY(launch_panel_t(t, pr[t], pcj[t], pw[t], abort_flag));
+                MODP_TRY(finish_t(t, KC, FAR, pr[t], pcj[t], pw[t], pw[t], abort_flag));
+                Pfull += pw[t];
+            }
+            MODP_TRY(block_end(KR, KC, FAR, Pfull, Pfull, np, abort_flag));
+            HIP_TRY(ctx, hipMemcpyAsync(hctl, bctl, sizeof(PanelCtl) * np, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipMemcpyAsync(hctl + MAXPAN, abort_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            const PanelCtl pc = *hctl;
-            if (pc.status == 3)
-                return set_error(ctx, MODP_ERR_NOT_INVERTIBLE, "pivot has no inverse modulo %lld (modulus not prime)",
-                                 (long long)p);
-            const int64_t npiv = pc.npiv;
-            if (npiv > 0) {
-                // the rest of the outer block: swaps; solve right of the panel
-                const int64_t nblk = (FAR - KC) - w;
-                if (nblk > 0) {
-                    TimedLaunch tl(ctx, "elim_finish");
-                    k_finish<<<grid_for(nblk * 64, 4096), 256, 0, ctx->stream>>>(
-                        W, ldw, KC, FAR, (int)rj, (int)cj, (int)w, (int)KC, pctl, pivinv, pivcol, colmap, mdst, msrc,
-                        Ubig, NBO, mp);
-                    HIP_TRY(ctx, hipGetLastError());
-                }
-                // G1: rest of the outer block, every row below the pivots
-                const int64_t r1 = rj + npiv;
-                MODP_TRY(gemm_sub(r1, m - r1, cj + w, FAR - (cj + w), cj, w, Ubig + (cj - KC) + (cj + w - KC) * NBO,
-                                  MODP_FORM_PLAIN_F64));
+            const int aborted = *(int *)(hctl + MAXPAN);
+            if (!aborted) {
+                kr += Pfull;
+                kc += Pfull;
+            } else {
+                int ts = 0;
+                while (ts < np && hctl[ts].status == 0) ++ts;
+                if (ts >= np)
+                    return set_error(ctx, MODP_ERR_DEVICE, "elimination pipeline aborted without a stopping panel");
+                if (hctl[ts].status == 3)
+                    return set_error(ctx, MODP_ERR_NOT_INVERTIBLE,
+                                     "pivot has no inverse modulo %lld (modulus not prime)", (long long)p);
+                const int64_t npiv = hctl[ts].npiv;
+                int64_t P = 0;
+                for (int t = 0; t < ts; ++t) P += pw[t];
+                if (npiv > 0) MODP_TRY(finish_t(ts, KC, FAR, pr[ts], pcj[ts], pw[ts], npiv, nullptr));
+                P += npiv;
+                MODP_TRY(block_end(KR, KC, FAR, P, P, ts + 1, nullptr));
+                kr += P;
+                kc += P;
+                stop_zero = hctl[ts].status == 1;
             }
-            kr += npiv;
-            kc += ple ? pc.ncols : npiv;
-            P += npiv;
-            if (pc.status == 1) { stop_zero = true; break; }
-            if (pc.status == 2) break;
-        }
-        const int64_t consumed = kc - KC;
-        if (P > 0) {
-            // columns outside the block: the block's swaps; far columns: solve
-            // with the block's L11, then the trailing GEMM with K = consumed
-            const int64_t nout = n - nbo;
-            if (nout > 0) {
-                TimedLaunch tl(ctx, "elim_block_finish");
-                int64_t g = (nout + 15) / 16;
-                if (g > 2048) g = 2048;
-                k_block_finish<<<(unsigned)g, 1024, 0, ctx->stream>>>(W, ldw, n, (int)KR, (int)KC, (int)FAR, (int)P,
-                                                                      (int)consumed, npanels, bmap_dst, bmap_src,
-                                                                      bctl, pivinv, pivcol, Ubig, NBO, mp);
-                HIP_TRY(ctx, hipGetLastError());
+        } else if (nbo > 0) {
+            // ple: panels skip zero columns, so the next panel's row depends on
+            // the pivots found; read each panel's outcome
+            int npanels = 0;
+            int64_t P = 0;
+            while (kc < FAR && kr < m) {
+                const int64_t cj = kc, rj = kr;
+                int64_t w = FAR - cj;
+                if (w > NBW) w = NBW;
+                MODP_TRY(launch_panel_t(npanels, rj, cj, w, nullptr));
+                HIP_TRY(ctx, hipMemcpyAsync(hctl, bctl + npanels, sizeof(PanelCtl), hipMemcpyDeviceToHost, ctx->stream));
+                HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                const PanelCtl pc = *hctl;
+                if (pc.status == 3)
+                    return set_error(ctx, MODP_ERR_NOT_INVERTIBLE,
+                                     "pivot has no inverse modulo %lld (modulus not prime)", (long long)p);
+                if (pc.npiv > 0) MODP_TRY(finish_t(npanels, KC, FAR, rj, cj, w, pc.npiv, nullptr));
+                ++npanels;
+                kr += pc.npiv;
+                kc += pc.ncols;
+                P += pc.npiv;
+                if (pc.status == 2) break;
             }
-            const int64_t rbeg = KR + P;
-            MODP_TRY(gemm_sub(rbeg, m - rbeg, FAR, n - FAR, KC, consumed, Ubig + (FAR - KC) * NBO, big_form));
+            MODP_TRY(block_end(KR, KC, FAR, P, kc - KC, npanels, nullptr));
         }
         if (!ple && kr < m && kc < n && (stop_zero || nbo == 0 || kc > last)) {
             // pluq zero column at kc (rows >= kr zero): whole block zero -> tail,

@@ -34,8 +34,10 @@ template <typename TA, typename TB, typename TC, int BM, int BN, int EPI>
 __global__ __launch_bounds__(256) void k_gemm_f64(int64_t m, int64_t n, int64_t k,
                                                   const TA *__restrict__ A, int64_t lda,
                                                   const TB *__restrict__ B, int64_t ldb,
-                                                  TC *__restrict__ C, int64_t ldc, ModP mp, int R)
+                                                  TC *__restrict__ C, int64_t ldc, ModP mp, int R,
+                                                  const int *__restrict__ skip)
 {
+    if (skip && *skip) return;
     __shared__ F64Smem<BM, BN> sm;
     gemm_f64_tile<TA, TB, TC, BM, BN, EPI>(m, n, k, A, lda, B, ldb, C, ldc, mp, R, (int64_t)blockIdx.y * BM,
                                            (int64_t)blockIdx.x * BN, sm);
@@ -53,8 +55,9 @@ constexpr int LROW = LBK + 16; // LDS row pitch in bytes
 template <typename TA>
 __global__ __launch_bounds__(256) void k_split_a(int64_t m, int64_t k, const TA *__restrict__ A, int64_t lda,
                                                  int8_t *__restrict__ Al, int64_t Mp, int64_t Kp, int L,
-                                                 ModP mp)
+                                                 ModP mp, const int *__restrict__ skip)
 {
+    if (skip && *skip) return;
     __shared__ uint32_t tile[64][65];
     const int tid = threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 64;
@@ -82,8 +85,9 @@ __global__ __launch_bounds__(256) void k_split_a(int64_t m, int64_t k, const TA 
 template <typename TB>
 __global__ __launch_bounds__(256) void k_split_b(int64_t k, int64_t n, const TB *__restrict__ B, int64_t ldb,
                                                  int8_t *__restrict__ Bl, int64_t Np, int64_t Kp, int L,
-                                                 ModP mp)
+                                                 ModP mp, const int *__restrict__ skip)
 {
+    if (skip && *skip) return;
     const int64_t chunks = Kp / 16;
     const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (e >= Np * chunks) return;
@@ -112,8 +116,10 @@ template <int L, int BM, int BN, typename TC, int EPI>
 __global__ __launch_bounds__(256) void k_gemm_limb(int64_t m, int64_t n, int64_t Kp,
                                                    const int8_t *__restrict__ Al, int64_t Mp,
                                                    const int8_t *__restrict__ Bl, int64_t Np,
-                                                   TC *__restrict__ C, int64_t ldc, ModP mp, LimbW lw)
+                                                   TC *__restrict__ C, int64_t ldc, ModP mp, LimbW lw,
+                                                   const int *__restrict__ skip)
 {
+    if (skip && *skip) return;
     constexpr int NS = 2 * L - 1;
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
@@ -247,15 +253,15 @@ int launch_f64_t(modp_ctx *ctx, const GemmArgs &g, int R)
     switch (g.epi) {
     case EPI_STORE:
         k_gemm_f64<TA, TB, TC, BM, BN, EPI_STORE><<<grid, 256, 0, ctx->stream>>>(
-            g.m, g.n, g.k, (const TA *)g.A, g.lda, (const TB *)g.B, g.ldb, (TC *)g.C, g.ldc, g.mp, R);
+            g.m, g.n, g.k, (const TA *)g.A, g.lda, (const TB *)g.B, g.ldb, (TC *)g.C, g.ldc, g.mp, R, g.skip);
         break;
     case EPI_SUB:
         k_gemm_f64<TA, TB, TC, BM, BN, EPI_SUB><<<grid, 256, 0, ctx->stream>>>(
-            g.m, g.n, g.k, (const TA *)g.A, g.lda, (const TB *)g.B, g.ldb, (TC *)g.C, g.ldc, g.mp, R);
+            g.m, g.n, g.k, (const TA *)g.A, g.lda, (const TB *)g.B, g.ldb, (TC *)g.C, g.ldc, g.mp, R, g.skip);
         break;
     default:
         k_gemm_f64<TA, TB, TC, BM, BN, EPI_ADD><<<grid, 256, 0, ctx->stream>>>(
-            g.m, g.n, g.k, (const TA *)g.A, g.lda, (const TB *)g.B, g.ldb, (TC *)g.C, g.ldc, g.mp, R);
+            g.m, g.n, g.k, (const TA *)g.A, g.lda, (const TB *)g.B, g.ldb, (TC *)g.C, g.ldc, g.mp, R, g.skip);
         break;
     }
     return hip_check(ctx, hipGetLastError(), "k_gemm_f64 launch");
@@ -292,7 +298,7 @@ int split_a_t(modp_ctx *ctx, const GemmArgs &g, const void *A, int64_t kbeg, int
 {
     dim3 grid((unsigned)(Kp / 64), (unsigned)(Mp / 64));
     const TA *Ap = (const TA *)A + kbeg * g.lda;
-    k_split_a<TA><<<grid, 256, 0, ctx->stream>>>(g.m, kc, Ap, g.lda, Al, Mp, Kp, L, g.mp);
+    k_split_a<TA><<<grid, 256, 0, ctx->stream>>>(g.m, kc, Ap, g.lda, Al, Mp, Kp, L, g.mp, g.skip);
     return hip_check(ctx, hipGetLastError(), "k_split_a launch");
 }
 
@@ -303,7 +309,7 @@ int split_b_t(modp_ctx *ctx, const GemmArgs &g, const void *B, int64_t kbeg, int
     int64_t total = Np * (Kp / 16);
     unsigned grid = (unsigned)((total + 255) / 256);
     const TB *Bp = (const TB *)B + kbeg;
-    k_split_b<TB><<<grid, 256, 0, ctx->stream>>>(kc, g.n, Bp, g.ldb, Bl, Np, Kp, L, g.mp);
+    k_split_b<TB><<<grid, 256, 0, ctx->stream>>>(kc, g.n, Bp, g.ldb, Bl, Np, Kp, L, g.mp, g.skip);
     return hip_check(ctx, hipGetLastError(), "k_split_b launch");
 }
 
@@ -314,11 +320,11 @@ int launch_limb_core(modp_ctx *ctx, const GemmArgs &g, int epi, const int8_t *Al
     dim3 grid((unsigned)(Np / BN), (unsigned)(Mp / BM));
     TimedLaunch tl(ctx, "gemm_limb");
     if (epi == EPI_STORE)
-        k_gemm_limb<L, BM, BN, TC, EPI_STORE><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, Kp, Al, Mp, Bl, Np, (TC *)g.C, g.ldc, g.mp, lw);
+        k_gemm_limb<L, BM, BN, TC, EPI_STORE><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, Kp, Al, Mp, Bl, Np, (TC *)g.C, g.ldc, g.mp, lw, g.skip);
     else if (epi == EPI_SUB)
-        k_gemm_limb<L, BM, BN, TC, EPI_SUB><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, Kp, Al, Mp, Bl, Np, (TC *)g.C, g.ldc, g.mp, lw);
+        k_gemm_limb<L, BM, BN, TC, EPI_SUB><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, Kp, Al, Mp, Bl, Np, (TC *)g.C, g.ldc, g.mp, lw, g.skip);
     else
-        k_gemm_limb<L, BM, BN, TC, EPI_ADD><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, Kp, Al, Mp, Bl, Np, (TC *)g.C, g.ldc, g.mp, lw);
+        k_gemm_limb<L, BM, BN, TC, EPI_ADD><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, Kp, Al, Mp, Bl, Np, (TC *)g.C, g.ldc, g.mp, lw, g.skip);
     return hip_check(ctx, hipGetLastError(), "k_gemm_limb launch");
 }
 
