@@ -94,6 +94,31 @@ __device__ __forceinline__ uint32_t modinv(uint32_t a, const ModP &mp)
     return (uint32_t)inv;
 }
 
+// XCD-aware tile order for a 1-D grid of 8 * ceil(tiles/8) workgroups.
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8), each with its own L2: give XCD x the contiguous range
+// [x*per, (x+1)*per) of the tile sequence, and walk that sequence in groups
+// of GROUP tile-rows so concurrent tiles of one XCD share A and B panels.
+// Returns false for the padding workgroups.
+__device__ __forceinline__ bool tile_of_block(int64_t tiles_m, int64_t tiles_n, int64_t *tm, int64_t *tn)
+{
+    constexpr int NXCD = 8, GROUP = 8;
+    const int64_t total = tiles_m * tiles_n;
+    const int64_t per = (total + NXCD - 1) / NXCD;
+    const int64_t b = blockIdx.x;
+    const int64_t t = (b % NXCD) * per + b / NXCD;
+    if (t >= total) return false;
+    const int64_t gsize = (int64_t)GROUP * tiles_n;
+    const int64_t g = t / gsize, first = g * GROUP;
+    const int64_t gm = tiles_m - first < GROUP ? tiles_m - first : GROUP;
+    const int64_t r = t % gsize;
+    *tm = first + r % gm;
+    *tn = r / gm;
+    return true;
+}
+
+static inline unsigned xcd_grid(int64_t tiles) { return (unsigned)(((tiles + 7) / 8) * 8); }
+
 // storage element -> residue in [0,p)
 template <typename T> __device__ __forceinline__ uint32_t to_res(T x, const ModP &mp);
 template <> __device__ __forceinline__ uint32_t to_res<double>(double x, const ModP &mp)

@@ -16,16 +16,6 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 namespace {
 
-template <typename T> __device__ __forceinline__ T ld_elem(const void *p, int64_t idx)
-{
-    return reinterpret_cast<const T *>(p)[idx];
-}
-
-template <int K> struct KindT;
-template <> struct KindT<EK_F64> { typedef double T; };
-template <> struct KindT<EK_F32> { typedef float T; };
-template <> struct KindT<EK_I32> { typedef int32_t T; };
-
 // --------------------------------------------------------------------------
 // plain FP64 form
 // --------------------------------------------------------------------------
@@ -38,9 +28,10 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int64_t m, int64_t n, int64_t 
                                                   const int *__restrict__ skip)
 {
     if (skip && *skip) return;
+    int64_t tm, tn;
+    if (!tile_of_block((m + BM - 1) / BM, (n + BN - 1) / BN, &tm, &tn)) return;
     __shared__ F64Smem<BM, BN> sm;
-    gemm_f64_tile<TA, TB, TC, BM, BN, EPI>(m, n, k, A, lda, B, ldb, C, ldc, mp, R, (int64_t)blockIdx.y * BM,
-                                           (int64_t)blockIdx.x * BN, sm);
+    gemm_f64_tile<TA, TB, TC, BM, BN, EPI>(m, n, k, A, lda, B, ldb, C, ldc, mp, R, tm * BM, tn * BN, sm);
 }
 
 // --------------------------------------------------------------------------
@@ -81,30 +72,32 @@ __global__ __launch_bounds__(256) void k_split_a(int64_t m, int64_t k, const TA 
     }
 }
 
-// B (k x n col-major) -> Bl[s][col][kk], zero-padded to Np x Kp.
+// B (k x n col-major) -> Bl[s][col][kk], zero-padded to Np x Kp.  Same
+// orientation (k contiguous), so each thread maps 4 consecutive k of one
+// column: consecutive threads read consecutive 32-byte runs.
 template <typename TB>
 __global__ __launch_bounds__(256) void k_split_b(int64_t k, int64_t n, const TB *__restrict__ B, int64_t ldb,
                                                  int8_t *__restrict__ Bl, int64_t Np, int64_t Kp, int L,
                                                  ModP mp, const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    const int64_t chunks = Kp / 16;
+    const int64_t quads = Kp / 4;
     const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (e >= Np * chunks) return;
-    const int64_t col = e / chunks, ch = e % chunks;
-    uint32_t v[16];
+    if (e >= Np * quads) return;
+    const int64_t col = e / quads, k0 = (e % quads) * 4;
+    uint32_t v[4];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        int64_t gk = ch * 16 + i;
+    for (int i = 0; i < 4; ++i) {
+        const int64_t gk = k0 + i;
         v[i] = (col < n && gk < k) ? to_res<TB>(B[gk + col * ldb], mp) : 0u;
     }
     const size_t plane = (size_t)Np * Kp;
-    int8_t *dst = Bl + (size_t)col * Kp + ch * 16;
+    int8_t *dst = Bl + (size_t)col * Kp + k0;
     for (int s = 0; s < L; ++s) {
-        union { int8_t b[16]; v4i v; } u;
+        uint32_t word = 0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) u.b[i] = (int8_t)((v[i] >> (7 * s)) & 127);
-        *reinterpret_cast<v4i *>(dst + s * plane) = u.v;
+        for (int i = 0; i < 4; ++i) word |= ((v[i] >> (7 * s)) & 127u) << (8 * i);
+        *reinterpret_cast<uint32_t *>(dst + s * plane) = word;
     }
 }
 
@@ -128,12 +121,14 @@ __global__ __launch_bounds__(256) void k_gemm_limb(int64_t m, int64_t n, int64_t
     __shared__ __attribute__((aligned(16))) int8_t As[L][BM][LROW];
     __shared__ __attribute__((aligned(16))) int8_t Bs[L][BN][LROW];
 
+    int64_t tm, tn;
+    if (!tile_of_block(Mp / BM, Np / BN, &tm, &tn)) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    const int64_t r0 = (int64_t)blockIdx.y * BM;
-    const int64_t c0 = (int64_t)blockIdx.x * BN;
+    const int64_t r0 = tm * BM;
+    const int64_t c0 = tn * BN;
     const size_t aplane = (size_t)Mp * Kp, bplane = (size_t)Np * Kp;
 
     v4i acc[FM][FN][NS];
@@ -248,7 +243,7 @@ int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 template <typename TA, typename TB, typename TC, int BM, int BN>
 int launch_f64_t(modp_ctx *ctx, const GemmArgs &g, int R)
 {
-    dim3 grid((unsigned)((g.n + BN - 1) / BN), (unsigned)((g.m + BM - 1) / BM));
+    const unsigned grid = xcd_grid(((g.n + BN - 1) / BN) * ((g.m + BM - 1) / BM));
     TimedLaunch tl(ctx, "gemm_f64");
     switch (g.epi) {
     case EPI_STORE:
@@ -306,7 +301,7 @@ template <typename TB>
 int split_b_t(modp_ctx *ctx, const GemmArgs &g, const void *B, int64_t kbeg, int64_t kc, int8_t *Bl,
               int64_t Np, int64_t Kp, int L)
 {
-    int64_t total = Np * (Kp / 16);
+    int64_t total = Np * (Kp / 4);
     unsigned grid = (unsigned)((total + 255) / 256);
     const TB *Bp = (const TB *)B + kbeg;
     k_split_b<TB><<<grid, 256, 0, ctx->stream>>>(kc, g.n, Bp, g.ldb, Bl, Np, Kp, L, g.mp, g.skip);
@@ -317,7 +312,7 @@ template <int L, int BM, int BN, typename TC>
 int launch_limb_core(modp_ctx *ctx, const GemmArgs &g, int epi, const int8_t *Al, int64_t Mp,
                      const int8_t *Bl, int64_t Np, int64_t Kp, const LimbW &lw)
 {
-    dim3 grid((unsigned)(Np / BN), (unsigned)(Mp / BM));
+    const unsigned grid = xcd_grid((Np / BN) * (Mp / BM));
     TimedLaunch tl(ctx, "gemm_limb");
     if (epi == EPI_STORE)
         k_gemm_limb<L, BM, BN, TC, EPI_STORE><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, Kp, Al, Mp, Bl, Np, (TC *)g.C, g.ldc, g.mp, lw, g.skip);
