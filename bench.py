@@ -300,12 +300,12 @@ def main():
     tfile = os.path.join(REPO, "profiles", "gemm_limb_8192_traffic.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
-    roof = {"bound": "mfma", "kernel": "k_gemm_limb<4,64,64>",
+    roof = {"bound": "mfma", "kernel": "k_gemm_limb<4 digits, 64x64 tile>",
             "achieved": round(achieved_tops, 2) if achieved_tops else None,
             "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
             "frac": round(achieved_tops / INT8_PEAK_TOPS, 4) if achieved_tops else None,
             "traffic": traffic,
-            "note": "achieved = 16 limb pairs x 2*8192^3 int8 ops per launch / mean launch time (HIP events); "
+            "note": "achieved = 16 digit pairs x 2*8192^3 int8 ops per launch / mean launch time (HIP events); "
                     "TFLOP/s here counts int8 ops (TOP/s)"}
 
     cpu = None
@@ -329,7 +329,8 @@ def main():
         "dtype": "int8",
         "data": "synthetic",
         "config": {"workload": f"mod-p GEMM {n}x{n}x{n}, p=2^26-5, residues uniform from seed 1, "
-                               "4x7-bit int8 limbs, int32 MFMA accumulation, FP64 Barrett recombine",
+                               "4 signed 8-bit limbs (base-256 digits) in int8, int32 MFMA accumulation, "
+                               "FP64 Barrett recombine",
                    "n": n, "p": p, "form": "limb_i8", "parallelism": "single GPU"},
         "wall_ms_per_step": round(limb_wall, 4),
         "gemm_core_ms": round(limb_kms, 4) if limb_kms else None,

@@ -200,7 +200,7 @@ int modp_ctx_destroy(modp_ctx *ctx)
     if (!ctx) return MODP_OK;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < WS_COUNT; ++i)
         if (ctx->ws[i]) hipFree(ctx->ws[i]);
     if (ctx->pinned) hipHostFree(ctx->pinned);
     if (ctx->invtab) hipFree(ctx->invtab);
@@ -302,10 +302,12 @@ int modp_eye(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int6
     return util_eye(ctx, kind, rows, cols, X, ldx);
 }
 
-int modp_gemm(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda,
-              const void *B, int64_t ldb, void *C, int64_t ldc, int64_t p)
+namespace {
+
+// argument checks shared by modp_gemm and modp_gemm_ex; fills g
+int gemm_prepare(modp_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda,
+                 const void *B, int64_t ldb, void *C, int64_t ldc, int64_t p, GemmArgs &g)
 {
-    CTX_CHECK(ctx);
     int kind;
     MODP_TRY(kind_of(ctx, dtype, &kind));
     MODP_TRY(check_mat(ctx, "A", A, m, k, lda));
@@ -314,27 +316,129 @@ int modp_gemm(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, int64_t 
     MODP_TRY(check_modulus_gemm(ctx, dtype, p));
     if (p > ((int64_t)1 << 26))
         return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld above 2^26", (long long)p);
-    if (form != MODP_FORM_AUTO && form != MODP_FORM_PLAIN_F64 && form != MODP_FORM_LIMB_I8)
-        return set_error(ctx, MODP_ERR_ARG, "unknown form %d", form);
-    if (m == 0 || n == 0) return MODP_OK;
-    GemmArgs g;
     g.m = m; g.n = n; g.k = k;
     g.A = A; g.lda = lda; g.ka = kind;
     g.B = B; g.ldb = ldb; g.kb = kind;
     g.C = C; g.ldc = ldc; g.kc = kind;
     g.epi = EPI_STORE;
     g.mp = make_modp(p);
-    if (k == 0) {
-        // empty contraction: C = 0
-        return util_eye(ctx, kind, 0, 0, C, ldc) == MODP_OK
-                   ? hip_check(ctx, hipMemset2DAsync(C, (size_t)ldc * (kind == EK_F64 ? 8 : 4), 0,
-                                                     (size_t)m * (kind == EK_F64 ? 8 : 4), (size_t)n, ctx->stream),
-                               "hipMemset2DAsync")
-                   : MODP_ERR_DEVICE;
-    }
-    int f = form == MODP_FORM_AUTO ? choose_form(m, n, k, p) : form;
-    if (f == MODP_FORM_LIMB_I8) return gemm_limb_i8(ctx, g);
+    return MODP_OK;
+}
+
+// C = 0 for an empty contraction
+int gemm_zero(modp_ctx *ctx, const GemmArgs &g)
+{
+    const size_t es = g.kc == EK_F64 ? 8 : 4;
+    return hip_check(ctx, hipMemset2DAsync(g.C, (size_t)g.ldc * es, 0, (size_t)g.m * es, (size_t)g.n, ctx->stream),
+                     "hipMemset2DAsync");
+}
+
+}  // namespace
+
+int modp_gemm(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda,
+              const void *B, int64_t ldb, void *C, int64_t ldc, int64_t p)
+{
+    CTX_CHECK(ctx);
+    GemmArgs g;
+    MODP_TRY(gemm_prepare(ctx, dtype, m, n, k, A, lda, B, ldb, C, ldc, p, g));
+    if (form != MODP_FORM_AUTO && form != MODP_FORM_PLAIN_F64 && form != MODP_FORM_LIMB_I8)
+        return set_error(ctx, MODP_ERR_ARG, "unknown form %d", form);
+    if (m == 0 || n == 0) return MODP_OK;
+    if (k == 0) return gemm_zero(ctx, g);
+    if (form == MODP_FORM_AUTO) return gemm_auto(ctx, g);
+    if (form == MODP_FORM_LIMB_I8) return gemm_limb_i8(ctx, g);
     return gemm_plain_f64(ctx, g);
+}
+
+int modp_gemm_ex(modp_ctx *ctx, const modp_gemm_config *cfg, int dtype, int64_t m, int64_t n, int64_t k,
+                 const void *A, int64_t lda, const void *B, int64_t ldb, void *C, int64_t ldc, int64_t p)
+{
+    CTX_CHECK(ctx);
+    if (!cfg) return set_error(ctx, MODP_ERR_ARG, "null configuration");
+    GemmArgs g;
+    MODP_TRY(gemm_prepare(ctx, dtype, m, n, k, A, lda, B, ldb, C, ldc, p, g));
+    GemmConfig c;
+    c.form = cfg->form; c.tile_m = cfg->tile_m; c.tile_n = cfg->tile_n; c.ksplit = cfg->ksplit;
+    if (!gemm_config_valid(g, c))
+        return set_error(ctx, MODP_ERR_ARG, "configuration (form %d, tile %dx%d, ksplit %d) invalid for modulus %lld",
+                         cfg->form, cfg->tile_m, cfg->tile_n, cfg->ksplit, (long long)p);
+    if (m == 0 || n == 0) return MODP_OK;
+    if (k == 0) return gemm_zero(ctx, g);
+    return gemm_run(ctx, g, c);
+}
+
+int modp_autotune_mode(modp_ctx *ctx, int mode)
+{
+    CTX_CHECK(ctx);
+    if (mode < MODP_TUNE_OFF || mode > MODP_TUNE_ON_MISS) return set_error(ctx, MODP_ERR_ARG, "unknown tune mode %d", mode);
+    ctx->tune_mode = mode;
+    return MODP_OK;
+}
+
+int modp_gemm_config_get(modp_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t k, int64_t p,
+                         modp_gemm_config *cfg, int *tuned)
+{
+    CTX_CHECK(ctx);
+    if (!cfg) return set_error(ctx, MODP_ERR_ARG, "null output");
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    if (m < 0 || n < 0 || k < 0) return set_error(ctx, MODP_ERR_DIM, "negative size");
+    MODP_TRY(check_modulus_gemm(ctx, dtype, p));
+    GemmArgs g;
+    g.m = m; g.n = n; g.k = k;
+    g.ka = g.kb = g.kc = kind;
+    g.mp = make_modp(p);
+    int t = 0;
+    GemmConfig c = gemm_lookup_config(ctx, g, &t);
+    cfg->form = c.form; cfg->tile_m = c.tile_m; cfg->tile_n = c.tile_n; cfg->ksplit = c.ksplit;
+    if (tuned) *tuned = t;
+    return MODP_OK;
+}
+
+int modp_autotune_gemm(modp_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t k, int64_t p,
+                       modp_gemm_config *best, double *best_ms)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    if (m <= 0 || n <= 0 || k <= 0) return set_error(ctx, MODP_ERR_DIM, "tuning needs positive sizes");
+    MODP_TRY(check_modulus_gemm(ctx, dtype, p));
+    if (p > ((int64_t)1 << 26)) return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld above 2^26", (long long)p);
+    const size_t es = kind == EK_F64 ? 8 : 4;
+    int st = MODP_OK;
+    void *A = ws_get(ctx, WS_TUNE_A, es * (size_t)m * (size_t)k, &st);
+    if (!A) return st;
+    void *B = ws_get(ctx, WS_TUNE_B, es * (size_t)k * (size_t)n, &st);
+    if (!B) return st;
+    void *C = ws_get(ctx, WS_TUNE_C, es * (size_t)m * (size_t)n, &st);
+    if (!C) return st;
+    MODP_TRY(util_rand(ctx, kind, m, k, A, m, p, 0x7475ULL, 0));
+    MODP_TRY(util_rand(ctx, kind, k, n, B, k, p, 0x7475ULL, (uint64_t)(m * k)));
+    GemmArgs g;
+    g.m = m; g.n = n; g.k = k;
+    g.A = A; g.lda = m; g.ka = kind;
+    g.B = B; g.ldb = k; g.kb = kind;
+    g.C = C; g.ldc = m; g.kc = kind;
+    g.epi = EPI_STORE;
+    g.mp = make_modp(p);
+    GemmConfig c;
+    double ms = 0;
+    MODP_TRY(tune_gemm(ctx, g, &c, &ms));
+    if (best) { best->form = c.form; best->tile_m = c.tile_m; best->tile_n = c.tile_n; best->ksplit = c.ksplit; }
+    if (best_ms) *best_ms = ms;
+    return MODP_OK;
+}
+
+int modp_autotune_export(modp_ctx *ctx, char *buf, size_t cap, size_t *len)
+{
+    CTX_CHECK(ctx);
+    return tune_export(ctx, buf, cap, len);
+}
+
+int modp_autotune_import(modp_ctx *ctx, const char *text)
+{
+    CTX_CHECK(ctx);
+    return tune_import(ctx, text);
 }
 
 int modp_pluq(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, const void *A, int64_t lda, void *U,

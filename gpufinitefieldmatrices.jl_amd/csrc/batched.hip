@@ -80,23 +80,116 @@ __global__ __launch_bounds__(256) void k_batched_gemm(const BJob *__restrict__ j
     }
 }
 
-// Tile choice per shape bucket.  Measured choices are cached per
-// (bucket of max(m,n), bucket of k); the default prefers 64x64 tiles unless
-// that leaves most of the tile empty.
-int pick_tile(int m, int n)
+// Tile choice per job: a 64x64 tile does 4x the MFMA work of a 32x32 tile per
+// queue slot and reuses each loaded operand twice as often; take it while its
+// padded work stays within thr times that of 32x32 tiles.  thr is the
+// autotuned knob of a plan (0: all 32x32, large: all 64x64).
+int pick_tile(int m, int n, double thr)
 {
     const int64_t w64 = (int64_t)((m + 63) / 64) * ((n + 63) / 64) * 64 * 64;
     const int64_t w32 = (int64_t)((m + 31) / 32) * ((n + 31) / 32) * 32 * 32;
-    // a 64-tile does 4x the MFMA work of a 32-tile per launch slot; accept
-    // up to 25% more padded work for the better operand reuse
-    return (double)w64 <= 1.25 * (double)w32 ? 64 : 32;
+    return (double)w64 <= thr * (double)w32 ? 64 : 32;
+}
+
+int ceil_log2(int64_t x)
+{
+    int b = 0;
+    while (((int64_t)1 << b) < x && b < 62) ++b;
+    return b;
+}
+
+int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count, double thr, modp_gemm_plan **out);
+
+double plan_time(modp_ctx *ctx, modp_gemm_plan *pl, int *st)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    double best = 1e30;
+    *st = hip_check(ctx, hipEventCreate(&a), "hipEventCreate");
+    if (*st == MODP_OK) *st = hip_check(ctx, hipEventCreate(&b), "hipEventCreate");
+    if (*st == MODP_OK) *st = batched_gemm_run(ctx, pl);
+    for (int r = 0; r < 3 && *st == MODP_OK; ++r) {
+        *st = hip_check(ctx, hipEventRecord(a, ctx->stream), "hipEventRecord");
+        if (*st == MODP_OK) *st = batched_gemm_run(ctx, pl);
+        if (*st == MODP_OK) *st = hip_check(ctx, hipEventRecord(b, ctx->stream), "hipEventRecord");
+        if (*st == MODP_OK) *st = hip_check(ctx, hipEventSynchronize(b), "hipEventSynchronize");
+        float t = 0.f;
+        if (*st == MODP_OK) *st = hip_check(ctx, hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
+        if (*st == MODP_OK) best = std::min(best, (double)t);
+    }
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    return best;
 }
 
 }  // namespace
 
+// Batch signature for the tuning cache: kind + 16, log2 buckets of the count
+// and of the mean m, n, k.
+TuneKey tune_key_batch(int kind, const modp_gemm_job *jobs, int64_t count)
+{
+    double sm = 0, sn = 0, sk = 0;
+    for (int64_t i = 0; i < count; ++i) { sm += jobs[i].m; sn += jobs[i].n; sk += jobs[i].k; }
+    const double c = count > 0 ? (double)count : 1.0;
+    TuneKey k;
+    k.v[0] = 16 + kind;
+    k.v[1] = ceil_log2(count);
+    k.v[2] = ceil_log2((int64_t)(sm / c + 0.5));
+    k.v[3] = ceil_log2((int64_t)(sn / c + 0.5));
+    k.v[4] = ceil_log2((int64_t)(sk / c + 0.5));
+    k.v[5] = 0;
+    k.v[6] = 0;
+    return k;
+}
+
+// The plan's tile policy comes from the tuning cache; on a miss with
+// MODP_TUNE_ON_MISS the candidate thresholds are built, each run (every run
+// rewrites the same outputs) and timed, and the fastest plan is kept.
 int batched_gemm_plan(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count, modp_gemm_plan **out)
 {
     if (kind == EK_I32) return set_error(ctx, MODP_ERR_DTYPE, "batched gemm: unsupported element type");
+    const double default_thr = 1.25;
+    if (ctx->tune_mode == MODP_TUNE_OFF || count == 0) return plan_build(ctx, kind, jobs, count, default_thr, out);
+    const TuneKey key = tune_key_batch(kind, jobs, count);
+    auto it = ctx->tune_cache.find(key);
+    if (it != ctx->tune_cache.end()) return plan_build(ctx, kind, jobs, count, it->second.thr, out);
+    if (ctx->tune_mode != MODP_TUNE_ON_MISS) return plan_build(ctx, kind, jobs, count, default_thr, out);
+    const double thrs[5] = {0.0, 1.0, 1.25, 1.6, 1e9};
+    modp_gemm_plan *best = nullptr;
+    double best_ms = 1e30, best_thr = default_thr;
+    for (double thr : thrs) {
+        modp_gemm_plan *pl = nullptr;
+        int st = plan_build(ctx, kind, jobs, count, thr, &pl);
+        if (st != MODP_OK) {
+            batched_gemm_destroy(ctx, best);
+            return st;
+        }
+        const double ms = plan_time(ctx, pl, &st);
+        if (st != MODP_OK) {
+            batched_gemm_destroy(ctx, pl);
+            batched_gemm_destroy(ctx, best);
+            return st;
+        }
+        if (ms < best_ms) {
+            batched_gemm_destroy(ctx, best);
+            best = pl;
+            best_ms = ms;
+            best_thr = thr;
+        } else {
+            batched_gemm_destroy(ctx, pl);
+        }
+    }
+    TuneEntry e;
+    e.thr = best_thr;
+    e.ms = best_ms;
+    ctx->tune_cache[key] = e;
+    *out = best;
+    return MODP_OK;
+}
+
+namespace {
+
+int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count, double thr, modp_gemm_plan **out)
+{
     modp_gemm_plan *pl = new modp_gemm_plan();
     pl->kind = kind;
     pl->count = count;
@@ -110,7 +203,7 @@ int batched_gemm_plan(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_
         d.m = (int)s.m; d.n = (int)s.n; d.k = (int)s.k;
         d.R = f64_reduce_interval(s.p);
         d.mp = make_modp(s.p);
-        d.tile = pick_tile(d.m, d.n);
+        d.tile = pick_tile(d.m, d.n, thr);
         d.tiles_n = (d.n + d.tile - 1) / d.tile;
     }
     // most expensive tiles first
@@ -160,6 +253,8 @@ int batched_gemm_plan(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_
     *out = pl;
     return MODP_OK;
 }
+
+}  // namespace
 
 int batched_gemm_run(modp_ctx *ctx, modp_gemm_plan *pl)
 {

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -150,14 +151,42 @@ struct KernelTiming {
     hipEvent_t a, b;
 };
 
+// GEMM launch configuration chosen by the heuristics or the autotuner
+struct GemmConfig {
+    int form = 0;     // MODP_FORM_PLAIN_F64 or MODP_FORM_LIMB_I8
+    int tile_m = 64;  // output tile: 64x64, 128x128, or (limb form) 128x64
+    int tile_n = 64;
+    int ksplit = 1;   // K slices run concurrently and summed mod p
+};
+
+// autotuner cache entries: key = (element kinds, bits of p, log2 buckets of
+// m, n, k) for single GEMMs; (kind, log2 buckets of count and mean m, n, k)
+// with kind + 16 for batched plans
+struct TuneKey {
+    int v[7];
+    bool operator<(const TuneKey &o) const
+    {
+        for (int i = 0; i < 7; ++i)
+            if (v[i] != o.v[i]) return v[i] < o.v[i];
+        return false;
+    }
+};
+struct TuneEntry {
+    GemmConfig cfg;     // single GEMM
+    double thr = 0.0;   // batched plan: 64-tile threshold
+    double ms = 0.0;    // measured time of the winner
+};
+
+constexpr int WS_COUNT = 12;
+
 struct modp_ctx {
     int device = 0;
     int num_cus = 0;
     hipStream_t stream = nullptr;
     std::string last_error;
     // grow-only device workspaces
-    void *ws[8] = {nullptr};
-    size_t ws_size[8] = {0};
+    void *ws[WS_COUNT] = {nullptr};
+    size_t ws_size[WS_COUNT] = {0};
     // pinned host staging for small device->host readbacks
     void *pinned = nullptr;
     size_t pinned_size = 0;
@@ -170,6 +199,9 @@ struct modp_ctx {
     uint32_t *invtab = nullptr;
     int64_t invtab_p = 0;
     size_t invtab_cap = 0;
+    // autotuner: MODP_TUNE_OFF / _CACHED / _ON_MISS and its cache
+    int tune_mode = 0;
+    std::map<TuneKey, TuneEntry> tune_cache;
 };
 
 // workspace slots
@@ -181,7 +213,11 @@ enum {
     WS_UBUF = 4,
     WS_MISC = 5,
     WS_BATCH = 6,
-    WS_SCRATCH = 7
+    WS_SCRATCH = 7,
+    WS_SLICES = 8,   // K-slice partial results
+    WS_TUNE_A = 9,   // synthetic operands of modp_autotune_gemm
+    WS_TUNE_B = 10,
+    WS_TUNE_C = 11
 };
 
 // error plumbing: every internal function returns a modp_status
@@ -232,10 +268,22 @@ struct GemmArgs {
     const int *skip = nullptr;
 };
 
+// heuristic configurations (gemm.hip)
 int gemm_plain_f64(modp_ctx *ctx, const GemmArgs &g);
 int gemm_limb_i8(modp_ctx *ctx, const GemmArgs &g);
 int limb_count(int64_t p);
 int choose_form(int64_t m, int64_t n, int64_t k, int64_t p);
+GemmConfig gemm_default_config(const modp_ctx *ctx, const GemmArgs &g, int form);
+bool gemm_config_valid(const GemmArgs &g, const GemmConfig &cfg);
+int gemm_run(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg);
+
+// autotuner (tune.hip)
+int gemm_auto(modp_ctx *ctx, const GemmArgs &g);
+int tune_gemm(modp_ctx *ctx, const GemmArgs &g, GemmConfig *best, double *best_ms);
+TuneKey tune_key_gemm(const GemmArgs &g);
+GemmConfig gemm_lookup_config(const modp_ctx *ctx, const GemmArgs &g, int *tuned);
+int tune_export(modp_ctx *ctx, char *buf, size_t cap, size_t *len);
+int tune_import(modp_ctx *ctx, const char *text);
 
 int util_mod(modp_ctx *ctx, int kind, int64_t rows, int64_t cols, void *X, int64_t ldx, int64_t N);
 int util_rand(modp_ctx *ctx, int kind, int64_t rows, int64_t cols, void *X, int64_t ldx, int64_t p,

@@ -6,13 +6,15 @@
 // reference's stripe bound, src/CuModMatrix/kernel_mul/stripe_mul.jl:13-27,
 // applied per MFMA step instead of per cuBLAS stripe).
 //
-// Limb form: residues split into L 7-bit limbs held in int8; the L*L limb
-// products run on v_mfma_i32_16x16x64_i8 with int32 accumulation, products of
-// equal shift s+t sharing one accumulator; K is cut into chunks so no int32
-// partial sum can overflow; the 2L-1 shift sums are reduced mod p and
-// recombined with weights 2^(7u) mod p in FP64.
+// Limb form (kernels in gemm_limb.h): residues as signed base-256 digits in
+// int8 on v_mfma_i32_16x16x64_i8; this file splits the operands into digit
+// planes, cuts K into int32-safe chunks and picks the launch.
+//
+// Both forms can cut K into slices that run concurrently (blockIdx.y) and are
+// summed mod p afterwards, for products whose output grid alone leaves CUs
+// idle.
+#include "gemm_limb.h"
 #include "gemm_tile.h"
-typedef int v4i __attribute__((ext_vector_type(4)));
 
 namespace {
 
@@ -20,220 +22,44 @@ namespace {
 // plain FP64 form
 // --------------------------------------------------------------------------
 
+// blockIdx.y selects a K slice [y*kslice, (y+1)*kslice); slice y writes its
+// residues at C + y*cslice (one slice: kslice >= k, cslice = 0).
 template <typename TA, typename TB, typename TC, int BM, int BN, int EPI>
 __global__ __launch_bounds__(256) void k_gemm_f64(int64_t m, int64_t n, int64_t k,
                                                   const TA *__restrict__ A, int64_t lda,
                                                   const TB *__restrict__ B, int64_t ldb,
                                                   TC *__restrict__ C, int64_t ldc, ModP mp, int R,
+                                                  int64_t kslice, int64_t cslice,
                                                   const int *__restrict__ skip)
 {
     if (skip && *skip) return;
     int64_t tm, tn;
     if (!tile_of_block((m + BM - 1) / BM, (n + BN - 1) / BN, &tm, &tn)) return;
+    const int64_t kb = (int64_t)blockIdx.y * kslice;
+    const int64_t klen = k - kb < kslice ? k - kb : kslice;
     __shared__ F64Smem<BM, BN> sm;
-    gemm_f64_tile<TA, TB, TC, BM, BN, EPI>(m, n, k, A, lda, B, ldb, C, ldc, mp, R, tm * BM, tn * BN, sm);
+    gemm_f64_tile<TA, TB, TC, BM, BN, EPI>(m, n, klen, A + kb * lda, lda, B + kb, ldb, C + blockIdx.y * cslice, ldc,
+                                           mp, R, tm * BM, tn * BN, sm);
 }
 
-// --------------------------------------------------------------------------
-// limb form
-// --------------------------------------------------------------------------
-
-constexpr int LBK = 64;        // k per MFMA / LDS stage
-constexpr int LROW = LBK + 16; // LDS row pitch in bytes
-
-// A (m x k col-major) -> Al[s][row][kk] (row-major per limb, k contiguous),
-// zero-padded to Mp x Kp.  64x64 tiles transposed through LDS.
-template <typename TA>
-__global__ __launch_bounds__(256) void k_split_a(int64_t m, int64_t k, const TA *__restrict__ A, int64_t lda,
-                                                 int8_t *__restrict__ Al, int64_t Mp, int64_t Kp, int L,
-                                                 ModP mp, const int *__restrict__ skip)
-{
-    if (skip && *skip) return;
-    __shared__ uint32_t tile[64][65];
-    const int tid = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 64;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        int row = tid & 63, kk = (tid >> 6) + 4 * i;
-        int64_t gr = r0 + row, gk = k0 + kk;
-        uint32_t v = 0;
-        if (gr < m && gk < k) v = to_res<TA>(A[gr + gk * lda], mp);
-        tile[row][kk] = v;
-    }
-    __syncthreads();
-    const int row = tid >> 2, ch = tid & 3;
-    const size_t plane = (size_t)Mp * Kp;
-    int8_t *dst = Al + (size_t)(r0 + row) * Kp + k0 + ch * 16;
-    for (int s = 0; s < L; ++s) {
-        union { int8_t b[16]; v4i v; } u;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) u.b[e] = (int8_t)((tile[row][ch * 16 + e] >> (7 * s)) & 127);
-        *reinterpret_cast<v4i *>(dst + s * plane) = u.v;
-    }
-}
-
-// B (k x n col-major) -> Bl[s][col][kk], zero-padded to Np x Kp.  Same
-// orientation (k contiguous), so each thread maps 4 consecutive k of one
-// column: consecutive threads read consecutive 32-byte runs.
-template <typename TB>
-__global__ __launch_bounds__(256) void k_split_b(int64_t k, int64_t n, const TB *__restrict__ B, int64_t ldb,
-                                                 int8_t *__restrict__ Bl, int64_t Np, int64_t Kp, int L,
-                                                 ModP mp, const int *__restrict__ skip)
-{
-    if (skip && *skip) return;
-    const int64_t quads = Kp / 4;
-    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (e >= Np * quads) return;
-    const int64_t col = e / quads, k0 = (e % quads) * 4;
-    uint32_t v[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int64_t gk = k0 + i;
-        v[i] = (col < n && gk < k) ? to_res<TB>(B[gk + col * ldb], mp) : 0u;
-    }
-    const size_t plane = (size_t)Np * Kp;
-    int8_t *dst = Bl + (size_t)col * Kp + k0;
-    for (int s = 0; s < L; ++s) {
-        uint32_t word = 0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) word |= ((v[i] >> (7 * s)) & 127u) << (8 * i);
-        *reinterpret_cast<uint32_t *>(dst + s * plane) = word;
-    }
-}
-
-struct LimbW {
-    double w[7];  // 2^(7u) mod p
-};
-
-template <int L, int BM, int BN, typename TC, int EPI>
-__global__ __launch_bounds__(256) void k_gemm_limb(int64_t m, int64_t n, int64_t Kp,
-                                                   const int8_t *__restrict__ Al, int64_t Mp,
-                                                   const int8_t *__restrict__ Bl, int64_t Np,
-                                                   TC *__restrict__ C, int64_t ldc, ModP mp, LimbW lw,
+// C = epi(C, sum of S slice results mod p); slice s holds m x n residues at
+// P + s*m*n (leading dimension m).
+template <typename TC, int EPI>
+__global__ __launch_bounds__(256) void k_slice_sum(int64_t m, int64_t n, const int32_t *__restrict__ P, int S,
+                                                   TC *__restrict__ C, int64_t ldc, ModP mp,
                                                    const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    constexpr int NS = 2 * L - 1;
-    constexpr int WM = BM / 2, WN = BN / 2;
-    constexpr int FM = WM / 16, FN = WN / 16;
-    constexpr int ACH = L * BM * (LBK / 16);  // 16-byte chunks of A per stage
-    constexpr int BCH = L * BN * (LBK / 16);
-    __shared__ __attribute__((aligned(16))) int8_t As[L][BM][LROW];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[L][BN][LROW];
-
-    int64_t tm, tn;
-    if (!tile_of_block(Mp / BM, Np / BN, &tm, &tn)) return;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int64_t r0 = tm * BM;
-    const int64_t c0 = tn * BN;
-    const size_t aplane = (size_t)Mp * Kp, bplane = (size_t)Np * Kp;
-
-    v4i acc[FM][FN][NS];
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j)
-#pragma unroll
-            for (int u = 0; u < NS; ++u) acc[i][j][u] = v4i{0, 0, 0, 0};
-
-    constexpr int AR = (ACH + 255) / 256, BR = (BCH + 255) / 256;
-    v4i ra[AR], rb[BR];
-    auto load = [&](int64_t k0) {
-#pragma unroll
-        for (int i = 0; i < AR; ++i) {
-            int idx = tid + 256 * i;
-            if (idx < ACH) {
-                int s = idx / (BM * 4), rem = idx % (BM * 4);
-                int row = rem >> 2, ch = rem & 3;
-                ra[i] = *reinterpret_cast<const v4i *>(Al + s * aplane + (size_t)(r0 + row) * Kp + k0 + ch * 16);
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < BR; ++i) {
-            int idx = tid + 256 * i;
-            if (idx < BCH) {
-                int s = idx / (BN * 4), rem = idx % (BN * 4);
-                int col = rem >> 2, ch = rem & 3;
-                rb[i] = *reinterpret_cast<const v4i *>(Bl + s * bplane + (size_t)(c0 + col) * Kp + k0 + ch * 16);
-            }
-        }
-    };
-    auto stash = [&]() {
-#pragma unroll
-        for (int i = 0; i < AR; ++i) {
-            int idx = tid + 256 * i;
-            if (idx < ACH) {
-                int s = idx / (BM * 4), rem = idx % (BM * 4);
-                *reinterpret_cast<v4i *>(&As[s][rem >> 2][(rem & 3) * 16]) = ra[i];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < BR; ++i) {
-            int idx = tid + 256 * i;
-            if (idx < BCH) {
-                int s = idx / (BN * 4), rem = idx % (BN * 4);
-                *reinterpret_cast<v4i *>(&Bs[s][rem >> 2][(rem & 3) * 16]) = rb[i];
-            }
-        }
-    };
-
-    const int64_t nk = Kp / LBK;
-    load(0);
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        stash();
-        __syncthreads();
-        if (kt + 1 < nk) load((kt + 1) * LBK);
-        v4i a[FM][L], b[FN][L];
-        const int ko = (lane >> 4) * 16;
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int s = 0; s < L; ++s)
-                a[i][s] = *reinterpret_cast<const v4i *>(&As[s][wm * WM + i * 16 + (lane & 15)][ko]);
-#pragma unroll
-        for (int j = 0; j < FN; ++j)
-#pragma unroll
-            for (int t = 0; t < L; ++t)
-                b[j][t] = *reinterpret_cast<const v4i *>(&Bs[t][wn * WN + j * 16 + (lane & 15)][ko]);
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j)
-#pragma unroll
-                for (int s = 0; s < L; ++s)
-#pragma unroll
-                    for (int t = 0; t < L; ++t)
-                        acc[i][j][s + t] =
-                            __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i][s], b[j][t], acc[i][j][s + t], 0, 0, 0);
-        __syncthreads();
-    }
-
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            int64_t gc = c0 + wn * WN + j * 16 + (lane & 15);
-            if (gc >= n) continue;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                int64_t gr = r0 + wm * WM + i * 16 + 4 * (lane >> 4) + e;
-                if (gr >= m) continue;
-                double tot = dmod((double)(uint32_t)acc[i][j][0][e], mp);
-#pragma unroll
-                for (int u = 1; u < NS; ++u) {
-                    double d = dmod((double)(uint32_t)acc[i][j][u][e], mp);
-                    tot = dmod(fma(d, lw.w[u], tot), mp);
-                }
-                uint32_t r = (uint32_t)tot;
-                TC *dst = C + gr + gc * ldc;
-                if (EPI == EPI_SUB) r = submod(to_res<TC>(*dst, mp), r, mp);
-                else if (EPI == EPI_ADD) r = addmod(to_res<TC>(*dst, mp), r, mp);
-                *dst = from_res<TC>(r);
-            }
-        }
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t mn = m * n;
+    if (idx >= mn) return;
+    double acc = 0.0;  // S * p < 2^53
+    for (int s = 0; s < S; ++s) acc += (double)P[(int64_t)s * mn + idx];
+    uint32_t r = (uint32_t)dmod(acc, mp);
+    TC *dst = C + idx % m + (idx / m) * ldc;
+    if (EPI == EPI_SUB) r = submod(to_res<TC>(*dst, mp), r, mp);
+    else if (EPI == EPI_ADD) r = addmod(to_res<TC>(*dst, mp), r, mp);
+    *dst = from_res<TC>(r);
 }
 
 int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -241,160 +67,186 @@ int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 // ---- dispatch helpers -----------------------------------------------------
 
 template <typename TA, typename TB, typename TC, int BM, int BN>
-int launch_f64_t(modp_ctx *ctx, const GemmArgs &g, int R)
+int launch_f64_t(modp_ctx *ctx, const GemmArgs &g, int R, const Slices &sl)
 {
-    const unsigned grid = xcd_grid(((g.n + BN - 1) / BN) * ((g.m + BM - 1) / BM));
+    const dim3 grid(xcd_grid(((g.n + BN - 1) / BN) * ((g.m + BM - 1) / BM)), (unsigned)sl.S);
     TimedLaunch tl(ctx, "gemm_f64");
+    const TA *A = (const TA *)g.A;
+    const TB *B = (const TB *)g.B;
+    TC *C = (TC *)g.C;
     switch (g.epi) {
     case EPI_STORE:
         k_gemm_f64<TA, TB, TC, BM, BN, EPI_STORE><<<grid, 256, 0, ctx->stream>>>(
-            g.m, g.n, g.k, (const TA *)g.A, g.lda, (const TB *)g.B, g.ldb, (TC *)g.C, g.ldc, g.mp, R, g.skip);
+            g.m, g.n, g.k, A, g.lda, B, g.ldb, C, g.ldc, g.mp, R, sl.kslice, sl.cslice, g.skip);
         break;
     case EPI_SUB:
         k_gemm_f64<TA, TB, TC, BM, BN, EPI_SUB><<<grid, 256, 0, ctx->stream>>>(
-            g.m, g.n, g.k, (const TA *)g.A, g.lda, (const TB *)g.B, g.ldb, (TC *)g.C, g.ldc, g.mp, R, g.skip);
+            g.m, g.n, g.k, A, g.lda, B, g.ldb, C, g.ldc, g.mp, R, sl.kslice, sl.cslice, g.skip);
         break;
     default:
         k_gemm_f64<TA, TB, TC, BM, BN, EPI_ADD><<<grid, 256, 0, ctx->stream>>>(
-            g.m, g.n, g.k, (const TA *)g.A, g.lda, (const TB *)g.B, g.ldb, (TC *)g.C, g.ldc, g.mp, R, g.skip);
+            g.m, g.n, g.k, A, g.lda, B, g.ldb, C, g.ldc, g.mp, R, sl.kslice, sl.cslice, g.skip);
         break;
     }
     return hip_check(ctx, hipGetLastError(), "k_gemm_f64 launch");
 }
 
 template <typename TA, typename TB, typename TC>
-int launch_f64_tile(modp_ctx *ctx, const GemmArgs &g, int R)
+int launch_f64_tile(modp_ctx *ctx, const GemmArgs &g, int R, int tile, const Slices &sl)
 {
-    // 128x128 tiles when the grid still fills the chip, else 64x64
-    int64_t t128 = ((g.m + 127) / 128) * ((g.n + 127) / 128);
-    if (t128 >= 2 * (int64_t)ctx->num_cus) return launch_f64_t<TA, TB, TC, 128, 128>(ctx, g, R);
-    return launch_f64_t<TA, TB, TC, 64, 64>(ctx, g, R);
+    if (tile == 128) return launch_f64_t<TA, TB, TC, 128, 128>(ctx, g, R, sl);
+    return launch_f64_t<TA, TB, TC, 64, 64>(ctx, g, R, sl);
 }
 
 template <typename TA, typename TB>
-int launch_f64_c(modp_ctx *ctx, const GemmArgs &g, int R)
+int launch_f64_c(modp_ctx *ctx, const GemmArgs &g, int R, int tile, const Slices &sl)
 {
-    if (g.kc == EK_F64) return launch_f64_tile<TA, TB, double>(ctx, g, R);
-    if (g.kc == EK_F32) return launch_f64_tile<TA, TB, float>(ctx, g, R);
-    return launch_f64_tile<TA, TB, int32_t>(ctx, g, R);
+    if (g.kc == EK_F64) return launch_f64_tile<TA, TB, double>(ctx, g, R, tile, sl);
+    if (g.kc == EK_F32) return launch_f64_tile<TA, TB, float>(ctx, g, R, tile, sl);
+    return launch_f64_tile<TA, TB, int32_t>(ctx, g, R, tile, sl);
 }
 
 template <typename TA>
-int launch_f64_b(modp_ctx *ctx, const GemmArgs &g, int R)
+int launch_f64_b(modp_ctx *ctx, const GemmArgs &g, int R, int tile, const Slices &sl)
 {
-    if (g.kb == EK_F64) return launch_f64_c<TA, double>(ctx, g, R);
-    if (g.kb == EK_F32) return launch_f64_c<TA, float>(ctx, g, R);
-    return launch_f64_c<TA, int32_t>(ctx, g, R);
+    if (g.kb == EK_F64) return launch_f64_c<TA, double>(ctx, g, R, tile, sl);
+    if (g.kb == EK_F32) return launch_f64_c<TA, float>(ctx, g, R, tile, sl);
+    return launch_f64_c<TA, int32_t>(ctx, g, R, tile, sl);
+}
+
+int launch_f64(modp_ctx *ctx, const GemmArgs &g, int R, int tile, const Slices &sl)
+{
+    if (g.ka == EK_F64) return launch_f64_b<double>(ctx, g, R, tile, sl);
+    if (g.ka == EK_F32) return launch_f64_b<float>(ctx, g, R, tile, sl);
+    return launch_f64_b<int32_t>(ctx, g, R, tile, sl);
+}
+
+template <typename TC>
+int launch_slice_sum_t(modp_ctx *ctx, const GemmArgs &g, int epi, const int32_t *P, int S)
+{
+    const int64_t mn = g.m * g.n;
+    const unsigned grid = (unsigned)((mn + 255) / 256);
+    TC *C = (TC *)g.C;
+    if (epi == EPI_STORE)
+        k_slice_sum<TC, EPI_STORE><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, P, S, C, g.ldc, g.mp, g.skip);
+    else if (epi == EPI_SUB)
+        k_slice_sum<TC, EPI_SUB><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, P, S, C, g.ldc, g.mp, g.skip);
+    else
+        k_slice_sum<TC, EPI_ADD><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, P, S, C, g.ldc, g.mp, g.skip);
+    return hip_check(ctx, hipGetLastError(), "k_slice_sum launch");
+}
+
+int launch_slice_sum(modp_ctx *ctx, const GemmArgs &g, int epi, const int32_t *P, int S)
+{
+    if (g.kc == EK_F64) return launch_slice_sum_t<double>(ctx, g, epi, P, S);
+    if (g.kc == EK_F32) return launch_slice_sum_t<float>(ctx, g, epi, P, S);
+    return launch_slice_sum_t<int32_t>(ctx, g, epi, P, S);
+}
+
+// slice results go to a workspace of S * m * n int32 residues
+int32_t *slice_buffer(modp_ctx *ctx, const GemmArgs &g, int S, int *st)
+{
+    return (int32_t *)ws_get(ctx, WS_SLICES, sizeof(int32_t) * (size_t)S * (size_t)g.m * (size_t)g.n, st);
 }
 
 template <typename TA>
 int split_a_t(modp_ctx *ctx, const GemmArgs &g, const void *A, int64_t kbeg, int64_t kc, int8_t *Al,
-              int64_t Mp, int64_t Kp, int L)
+              int64_t Mp, int64_t Kp, int L, int32_t top)
 {
     dim3 grid((unsigned)(Kp / 64), (unsigned)(Mp / 64));
     const TA *Ap = (const TA *)A + kbeg * g.lda;
-    k_split_a<TA><<<grid, 256, 0, ctx->stream>>>(g.m, kc, Ap, g.lda, Al, Mp, Kp, L, g.mp, g.skip);
+    k_split_a<TA><<<grid, 256, 0, ctx->stream>>>(g.m, kc, Ap, g.lda, Al, Mp, Kp, L, top, g.mp, g.skip);
     return hip_check(ctx, hipGetLastError(), "k_split_a launch");
 }
 
 template <typename TB>
 int split_b_t(modp_ctx *ctx, const GemmArgs &g, const void *B, int64_t kbeg, int64_t kc, int8_t *Bl,
-              int64_t Np, int64_t Kp, int L)
+              int64_t Np, int64_t Kp, int L, int32_t top)
 {
     int64_t total = Np * (Kp / 4);
     unsigned grid = (unsigned)((total + 255) / 256);
     const TB *Bp = (const TB *)B + kbeg;
-    k_split_b<TB><<<grid, 256, 0, ctx->stream>>>(kc, g.n, Bp, g.ldb, Bl, Np, Kp, L, g.mp, g.skip);
+    k_split_b<TB><<<grid, 256, 0, ctx->stream>>>(kc, g.n, Bp, g.ldb, Bl, Np, Kp, L, top, g.mp, g.skip);
     return hip_check(ctx, hipGetLastError(), "k_split_b launch");
 }
 
-template <int L, int BM, int BN, typename TC>
-int launch_limb_core(modp_ctx *ctx, const GemmArgs &g, int epi, const int8_t *Al, int64_t Mp,
-                     const int8_t *Bl, int64_t Np, int64_t Kp, const LimbW &lw)
-{
-    const unsigned grid = xcd_grid((Np / BN) * (Mp / BM));
-    TimedLaunch tl(ctx, "gemm_limb");
-    if (epi == EPI_STORE)
-        k_gemm_limb<L, BM, BN, TC, EPI_STORE><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, Kp, Al, Mp, Bl, Np, (TC *)g.C, g.ldc, g.mp, lw, g.skip);
-    else if (epi == EPI_SUB)
-        k_gemm_limb<L, BM, BN, TC, EPI_SUB><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, Kp, Al, Mp, Bl, Np, (TC *)g.C, g.ldc, g.mp, lw, g.skip);
-    else
-        k_gemm_limb<L, BM, BN, TC, EPI_ADD><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, Kp, Al, Mp, Bl, Np, (TC *)g.C, g.ldc, g.mp, lw, g.skip);
-    return hip_check(ctx, hipGetLastError(), "k_gemm_limb launch");
-}
-
-template <int L, typename TC>
-int launch_limb_l(modp_ctx *ctx, const GemmArgs &g, int epi, const int8_t *Al, int64_t Mp,
-                  const int8_t *Bl, int64_t Np, int64_t Kp, const LimbW &lw, int tile)
-{
-    if (tile == 128) return launch_limb_core<L, 128, 128, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw);
-    return launch_limb_core<L, 64, 64, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw);
-}
-
-template <typename TC>
-int launch_limb_c(modp_ctx *ctx, const GemmArgs &g, int L, int epi, const int8_t *Al, int64_t Mp,
-                  const int8_t *Bl, int64_t Np, int64_t Kp, const LimbW &lw, int tile)
+int launch_limb(modp_ctx *ctx, const GemmArgs &g, int L, int epi, const int8_t *Al, int64_t Mp,
+                const int8_t *Bl, int64_t Np, int64_t Kp, const LimbW &lw, int tm, int tn, const Slices &sl)
 {
     switch (L) {
-    case 1: return launch_limb_l<1, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, tile);
-    case 2: return launch_limb_l<2, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, tile);
-    case 3: return launch_limb_l<3, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, 64);
-    default: return launch_limb_l<4, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, 64);
+    case 1: return launch_limb_impl<1>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, tm, tn, sl);
+    case 2: return launch_limb_impl<2>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, tm, tn, sl);
+    case 3: return launch_limb_impl<3>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, tm, tn, sl);
+    default: return launch_limb_impl<4>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, tm, tn, sl);
     }
 }
 
-}  // namespace
+int64_t tiles_of(const GemmArgs &g, int tm, int tn) { return ((g.m + tm - 1) / tm) * ((g.n + tn - 1) / tn); }
 
-int limb_count(int64_t p)
+// slices of a K extent: S0 requested, each a multiple of `align` and at
+// least `minlen` long
+Slices make_slices(int64_t kext, int S0, int64_t align, int64_t minlen)
 {
-    int bits = 0;
-    int64_t x = p - 1;
-    while (x > 0) { ++bits; x >>= 1; }
-    int L = (bits + 6) / 7;
-    return L < 1 ? 1 : L;
+    Slices sl;
+    if (S0 <= 1 || kext <= minlen) {
+        sl.S = 1;
+        sl.kslice = kext > 0 ? kext : 1;
+        return sl;
+    }
+    int64_t ks = round_up((kext + S0 - 1) / S0, align);
+    if (ks < minlen) ks = round_up(minlen, align);
+    sl.S = (int)((kext + ks - 1) / ks);
+    sl.kslice = ks;
+    return sl;
 }
 
-int choose_form(int64_t m, int64_t n, int64_t k, int64_t p)
-{
-    (void)m; (void)n; (void)p;
-    // the limb form pays a split pass over A and B: worth it once the
-    // contraction is deep enough for MFMA time to dominate
-    if (k >= 256) return MODP_FORM_LIMB_I8;
-    return MODP_FORM_PLAIN_F64;
-}
-
-int gemm_plain_f64(modp_ctx *ctx, const GemmArgs &g)
+int gemm_plain_cfg(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg)
 {
     if (g.m <= 0 || g.n <= 0) return MODP_OK;
     const int R = f64_reduce_interval(g.mp.p);
     if (R < 1) return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld too large for the FP64 form", (long long)g.mp.p);
-    if (g.ka == EK_F64) return launch_f64_b<double>(ctx, g, R);
-    if (g.ka == EK_F32) return launch_f64_b<float>(ctx, g, R);
-    return launch_f64_b<int32_t>(ctx, g, R);
+    const int tile = cfg.tile_m == 128 ? 128 : 64;
+    Slices sl = make_slices(g.k, cfg.ksplit, PBK, 64);
+    if (sl.S == 1) {
+        sl.kslice = g.k > 0 ? g.k : 1;
+        return launch_f64(ctx, g, R, tile, sl);
+    }
+    int st = MODP_OK;
+    int32_t *P = slice_buffer(ctx, g, sl.S, &st);
+    if (!P) return st;
+    GemmArgs gs = g;
+    gs.C = P; gs.ldc = g.m; gs.kc = EK_I32; gs.epi = EPI_STORE;
+    sl.cslice = g.m * g.n;
+    MODP_TRY(launch_f64(ctx, gs, R, tile, sl));
+    return launch_slice_sum(ctx, g, g.epi, P, sl.S);
 }
 
-int gemm_limb_i8(modp_ctx *ctx, const GemmArgs &g)
+int gemm_limb_cfg(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg)
 {
     if (g.m <= 0 || g.n <= 0) return MODP_OK;
     const int L = limb_count(g.mp.p);
     if (L > 4) return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld needs more than 4 limbs", (long long)g.mp.p);
-    // largest k-chunk whose shift sums fit int32: pairs(max) * K * 127^2 < 2^31
-    const int64_t pairs = L;  // shift L-1 collects L pairs
-    int64_t kmax = ((int64_t)2147483647 / (pairs * 127 * 127)) / LBK * LBK;
-    const int tile = (L <= 2 && ((g.m + 127) / 128) * ((g.n + 127) / 128) >= 2 * (int64_t)ctx->num_cus) ? 128 : 64;
-    const int64_t Mp = round_up(g.m, tile), Np = round_up(g.n, tile);
+    // largest k-chunk whose shift sums fit int32: a digit product is at most
+    // 128^2 in magnitude and shift L-1 collects L of them per k, so
+    // L * K * 2^14 <= 2^31 - 1
+    const int64_t pairs = L;
+    int64_t kmax = ((int64_t)2147483647 / (pairs * 16384)) / LBK * LBK;
+    // residues above top are represented as r - p (see digit_rep)
+    int64_t S = 0;
+    for (int u = 0; u < L; ++u) S = S * 256 + 1;
+    const int32_t top = (int32_t)(127 * S > 2147483647 ? 2147483647 : 127 * S);
+    const int tm = cfg.tile_m == 128 ? 128 : 64;
+    const int tn = (cfg.tile_m == 128 && cfg.tile_n == 128 && L <= 2) ? 128 : 64;
+    const int64_t Mp = round_up(g.m, tm), Np = round_up(g.n, tn);
     LimbW lw;
     {
         double w = 1.0;
         for (int u = 0; u < 7; ++u) {
             lw.w[u] = w;
-            w = fmod(w * 128.0, (double)g.mp.p);
+            w = fmod(w * 256.0, (double)g.mp.p);
         }
     }
-    if (g.k <= 0) {
-        // empty contraction: C = 0 (STORE) or unchanged
-        if (g.epi != EPI_STORE) return MODP_OK;
-    }
+    if (g.k <= 0 && g.epi != EPI_STORE) return MODP_OK;  // C unchanged
     int64_t kdone = 0;
     bool first = true;
     do {
@@ -406,19 +258,108 @@ int gemm_limb_i8(modp_ctx *ctx, const GemmArgs &g)
         if (!Al) return st;
         int8_t *Bl = (int8_t *)ws_get(ctx, WS_BLIMB, (size_t)L * Np * Kp, &st);
         if (!Bl) return st;
-        if (g.ka == EK_F64) MODP_TRY(split_a_t<double>(ctx, g, g.A, kdone, kc, Al, Mp, Kp, L));
-        else if (g.ka == EK_F32) MODP_TRY(split_a_t<float>(ctx, g, g.A, kdone, kc, Al, Mp, Kp, L));
-        else MODP_TRY(split_a_t<int32_t>(ctx, g, g.A, kdone, kc, Al, Mp, Kp, L));
-        if (g.kb == EK_F64) MODP_TRY(split_b_t<double>(ctx, g, g.B, kdone, kc, Bl, Np, Kp, L));
-        else if (g.kb == EK_F32) MODP_TRY(split_b_t<float>(ctx, g, g.B, kdone, kc, Bl, Np, Kp, L));
-        else MODP_TRY(split_b_t<int32_t>(ctx, g, g.B, kdone, kc, Bl, Np, Kp, L));
+        if (g.ka == EK_F64) MODP_TRY(split_a_t<double>(ctx, g, g.A, kdone, kc, Al, Mp, Kp, L, top));
+        else if (g.ka == EK_F32) MODP_TRY(split_a_t<float>(ctx, g, g.A, kdone, kc, Al, Mp, Kp, L, top));
+        else MODP_TRY(split_a_t<int32_t>(ctx, g, g.A, kdone, kc, Al, Mp, Kp, L, top));
+        if (g.kb == EK_F64) MODP_TRY(split_b_t<double>(ctx, g, g.B, kdone, kc, Bl, Np, Kp, L, top));
+        else if (g.kb == EK_F32) MODP_TRY(split_b_t<float>(ctx, g, g.B, kdone, kc, Bl, Np, Kp, L, top));
+        else MODP_TRY(split_b_t<int32_t>(ctx, g, g.B, kdone, kc, Bl, Np, Kp, L, top));
         int epi = g.epi;
         if (!first && epi == EPI_STORE) epi = EPI_ADD;
-        if (g.kc == EK_F64) MODP_TRY(launch_limb_c<double>(ctx, g, L, epi, Al, Mp, Bl, Np, Kp, lw, tile));
-        else if (g.kc == EK_F32) MODP_TRY(launch_limb_c<float>(ctx, g, L, epi, Al, Mp, Bl, Np, Kp, lw, tile));
-        else MODP_TRY(launch_limb_c<int32_t>(ctx, g, L, epi, Al, Mp, Bl, Np, Kp, lw, tile));
+        Slices sl = make_slices(Kp, cfg.ksplit, LBK, 256);
+        if (sl.S == 1) {
+            sl.kslice = Kp;
+            MODP_TRY(launch_limb(ctx, g, L, epi, Al, Mp, Bl, Np, Kp, lw, tm, tn, sl));
+        } else {
+            int32_t *P = slice_buffer(ctx, g, sl.S, &st);
+            if (!P) return st;
+            GemmArgs gs = g;
+            gs.C = P; gs.ldc = g.m; gs.kc = EK_I32;
+            sl.cslice = g.m * g.n;
+            MODP_TRY(launch_limb(ctx, gs, L, EPI_STORE, Al, Mp, Bl, Np, Kp, lw, tm, tn, sl));
+            MODP_TRY(launch_slice_sum(ctx, g, epi, P, sl.S));
+        }
         kdone += kc;
         first = false;
     } while (kdone < g.k);
     return MODP_OK;
+}
+
+}  // namespace
+
+int limb_count(int64_t p)
+{
+    // signed base-256 digits: L digits represent every residue class iff p <= 256^L
+    int L = 1;
+    while (L < 8 && p > ((int64_t)1 << (8 * L))) ++L;
+    return L;
+}
+
+int choose_form(int64_t m, int64_t n, int64_t k, int64_t p)
+{
+    (void)m; (void)n; (void)p;
+    // the limb form pays a split pass over A and B: worth it once the
+    // contraction is deep enough for MFMA time to dominate
+    if (k >= 256) return MODP_FORM_LIMB_I8;
+    return MODP_FORM_PLAIN_F64;
+}
+
+bool gemm_config_valid(const GemmArgs &g, const GemmConfig &cfg)
+{
+    if (cfg.ksplit < 1 || cfg.ksplit > 16) return false;
+    const bool sq64 = cfg.tile_m == 64 && cfg.tile_n == 64;
+    const bool sq128 = cfg.tile_m == 128 && cfg.tile_n == 128;
+    const bool r128x64 = cfg.tile_m == 128 && cfg.tile_n == 64;
+    if (cfg.form == MODP_FORM_PLAIN_F64) return (sq64 || sq128) && f64_reduce_interval(g.mp.p) >= 1;
+    // 128x128 limb tiles hold 2L-1 accumulators of a 64x64 block per wave:
+    // register room for L <= 2 only
+    if (cfg.form == MODP_FORM_LIMB_I8) {
+        const int L = limb_count(g.mp.p);
+        return (sq64 || r128x64 || (sq128 && L <= 2)) && L <= 4;
+    }
+    return false;
+}
+
+GemmConfig gemm_default_config(const modp_ctx *ctx, const GemmArgs &g, int form)
+{
+    GemmConfig c;
+    c.form = form == MODP_FORM_AUTO ? choose_form(g.m, g.n, g.k, g.mp.p) : form;
+    c.tile_m = c.tile_n = 64;
+    // larger tiles while the grid still fills the chip: measured best at
+    // n = 8192 are 128x128 (plain; limb with one digit) and 128x64 (limb with
+    // two digits); three and four digits gain nothing over 64x64
+    if (tiles_of(g, 128, 128) >= 2 * ctx->num_cus) {
+        const int L = limb_count(g.mp.p);
+        if (c.form == MODP_FORM_PLAIN_F64 || L == 1) {
+            c.tile_m = c.tile_n = 128;
+        } else if (L == 2) {
+            c.tile_m = 128;
+        }
+    }
+    // a grid far below one workgroup per CU: cut K into concurrent slices
+    const int64_t t = tiles_of(g, c.tile_m, c.tile_n);
+    c.ksplit = 1;
+    if (2 * t <= ctx->num_cus && g.k >= 2048) {
+        int64_t S = ctx->num_cus / (t > 0 ? t : 1);
+        if (S > 8) S = 8;
+        while (S > 1 && g.k / S < 512) --S;
+        c.ksplit = (int)S;
+    }
+    return c;
+}
+
+int gemm_run(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg)
+{
+    if (cfg.form == MODP_FORM_LIMB_I8) return gemm_limb_cfg(ctx, g, cfg);
+    return gemm_plain_cfg(ctx, g, cfg);
+}
+
+int gemm_plain_f64(modp_ctx *ctx, const GemmArgs &g)
+{
+    return gemm_plain_cfg(ctx, g, gemm_default_config(ctx, g, MODP_FORM_PLAIN_F64));
+}
+
+int gemm_limb_i8(modp_ctx *ctx, const GemmArgs &g)
+{
+    return gemm_limb_cfg(ctx, g, gemm_default_config(ctx, g, MODP_FORM_LIMB_I8));
 }

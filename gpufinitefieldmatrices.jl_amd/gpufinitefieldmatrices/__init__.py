@@ -29,6 +29,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import FORM_AUTO, FORM_LIMB_I8, FORM_PLAIN_F64, GemmJob, ElimJob
+from ._lib import TUNE_CACHED, TUNE_OFF, TUNE_ON_MISS
 
 TILE_WIDTH = 32
 DEFAULT_TYPE = np.float32
@@ -141,6 +142,42 @@ class Context:
         ms = ctypes.c_float()
         self.check(self.lib.modp_event_elapsed(self.h, a, b, ctypes.byref(ms)))
         return ms.value
+
+    # autotuner (no counterpart in the reference)
+    def autotune_mode(self, mode: int):
+        """TUNE_OFF (heuristics), TUNE_CACHED, or TUNE_ON_MISS."""
+        self.check(self.lib.modp_autotune_mode(self.h, int(mode)), "autotune_mode")
+
+    def autotune_gemm(self, m: int, n: int, k: int, N: int, elem_type=np.float64):
+        """Time every valid launch configuration on synthetic operands; the
+        fastest is cached and returned as (config dict, milliseconds)."""
+        cfg = _lib.GemmConfig()
+        ms = ctypes.c_double()
+        self.check(self.lib.modp_autotune_gemm(self.h, _dtype_code(elem_type), m, n, k, int(N), ctypes.byref(cfg),
+                                               ctypes.byref(ms)), "autotune_gemm")
+        return _cfg_dict(cfg), ms.value
+
+    def gemm_config(self, m: int, n: int, k: int, N: int, elem_type=np.float64):
+        """(config dict, tuned?) that a FORM_AUTO product of this shape runs."""
+        cfg = _lib.GemmConfig()
+        tuned = ctypes.c_int()
+        self.check(self.lib.modp_gemm_config_get(self.h, _dtype_code(elem_type), m, n, k, int(N), ctypes.byref(cfg),
+                                                 ctypes.byref(tuned)), "gemm_config")
+        return _cfg_dict(cfg), bool(tuned.value)
+
+    def autotune_export(self) -> str:
+        n = ctypes.c_size_t()
+        self.check(self.lib.modp_autotune_export(self.h, None, 0, ctypes.byref(n)), "autotune_export")
+        buf = ctypes.create_string_buffer(n.value)
+        self.check(self.lib.modp_autotune_export(self.h, buf, n.value, ctypes.byref(n)), "autotune_export")
+        return buf.value.decode()
+
+    def autotune_import(self, text: str):
+        self.check(self.lib.modp_autotune_import(self.h, text.encode()), "autotune_import")
+
+
+def _cfg_dict(cfg) -> dict:
+    return {"form": cfg.form, "tile_m": cfg.tile_m, "tile_n": cfg.tile_n, "ksplit": cfg.ksplit}
 
 
 _CTX: Optional[Context] = None
@@ -397,19 +434,27 @@ def find_max_stripe_ops(elem_type, N: int) -> int:
     return (2 ** bits - 1) // ((N - 1) ** 2) - 1
 
 
-def _gemm_raw(C: CuModArray, A: CuModArray, B: CuModArray, N: int, form: int = FORM_AUTO):
+def _gemm_raw(C: CuModArray, A: CuModArray, B: CuModArray, N: int, form: int = FORM_AUTO, config=None):
     m, k = A.size
     n = B.size[1]
     ctx = C.ctx
+    if config is not None:
+        cfg = _lib.GemmConfig(**dict(config))
+        ctx.check(ctx.lib.modp_gemm_ex(ctx.h, ctypes.byref(cfg), C.dtype_code, m, n, k, A.ptr, A.ld, B.ptr, B.ld,
+                                       C.ptr, C.ld, int(N)), "gemm")
+        return
     ctx.check(ctx.lib.modp_gemm(ctx.h, form, C.dtype_code, m, n, k, A.ptr, A.ld, B.ptr, B.ld, C.ptr, C.ld, int(N)),
               "gemm")
 
 
-def stripe_mul_(C: CuModArray, A: CuModArray, B: CuModArray, M=None, N=None, form: int = FORM_AUTO) -> None:
+def stripe_mul_(C: CuModArray, A: CuModArray, B: CuModArray, M=None, N=None, form: int = FORM_AUTO,
+                config=None) -> None:
     """stripe_mul!(C, A, B; M, N) (stripe_mul.jl:175-244): C = A*B mod N.
 
     The device kernels keep every partial sum exact for any N <= 2^26, so
-    the stripe width M is only used for the reference's range check."""
+    the stripe width M is only used for the reference's range check.
+    ``config`` (a dict with keys form, tile_m, tile_n, ksplit) forces a
+    launch configuration instead of the form / autotuner choice."""
     if A.N != B.N or B.N != C.N:
         raise ArgumentError("Mismatched modulus in matmul")
     if cols(A) != rows(B):
@@ -424,10 +469,11 @@ def stripe_mul_(C: CuModArray, A: CuModArray, B: CuModArray, M=None, N=None, for
     if M < 1:
         raise ArgumentError(f"cannot perform a single multiplication for modulus {A.N} with datatype "
                             f"{'Float64' if A.elem_type == np.float64 else 'Float32'}")
-    _gemm_raw(C, A, B, C.N if N is None else R, form)
+    _gemm_raw(C, A, B, C.N if N is None else R, form, config)
 
 
-def mul_(C: CuModArray, A: CuModArray, B: CuModArray, M=None, N=None, form: int = FORM_AUTO) -> CuModArray:
+def mul_(C: CuModArray, A: CuModArray, B: CuModArray, M=None, N=None, form: int = FORM_AUTO,
+         config=None) -> CuModArray:
     """LinearAlgebra.mul!(C, A, B; M, N) (CuModMatrix.jl:767-787)."""
     if A.N != B.N or A.N != C.N:
         raise CuModArrayModulusMismatchException("All matrices must have the same modulus N")
@@ -435,7 +481,7 @@ def mul_(C: CuModArray, A: CuModArray, B: CuModArray, M=None, N=None, form: int 
         raise CuModArraySizeMismatchException("Matrix dimensions do not match for multiplication")
     if rows(C) != rows(A) or cols(C) != cols(B):
         raise CuModArraySizeMismatchException("Output matrix C has incorrect dimensions")
-    stripe_mul_(C, A, B, M=M, N=N, form=form)
+    stripe_mul_(C, A, B, M=M, N=N, form=form, config=config)
     return C
 
 

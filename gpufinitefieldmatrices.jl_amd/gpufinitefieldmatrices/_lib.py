@@ -36,6 +36,10 @@ FORM_AUTO = 0
 FORM_PLAIN_F64 = 1
 FORM_LIMB_I8 = 2
 
+TUNE_OFF = 0
+TUNE_CACHED = 1
+TUNE_ON_MISS = 2
+
 ELIM_PLUQ = 0
 ELIM_LU = 1
 ELIM_RREF = 2
@@ -45,6 +49,11 @@ class GemmJob(ctypes.Structure):
     _fields_ = [("m", _i64), ("n", _i64), ("k", _i64), ("p", _i64),
                 ("A", _vp), ("lda", _i64), ("B", _vp), ("ldb", _i64),
                 ("C", _vp), ("ldc", _i64)]
+
+
+class GemmConfig(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_int32), ("tile_m", ctypes.c_int32), ("tile_n", ctypes.c_int32),
+                ("ksplit", ctypes.c_int32)]
 
 
 class ElimJob(ctypes.Structure):
@@ -73,6 +82,15 @@ SIGNATURES = [
     ("modp_rand", _int, [_vp, _int, _i64, _i64, _vp, _i64, _i64, _u64, _u64]),
     ("modp_eye", _int, [_vp, _int, _i64, _i64, _vp, _i64]),
     ("modp_gemm", _int, [_vp, _int, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64]),
+    ("modp_autotune_mode", _int, [_vp, _int]),
+    ("modp_autotune_gemm", _int, [_vp, _int, _i64, _i64, _i64, _i64, ctypes.POINTER(GemmConfig),
+                                  ctypes.POINTER(ctypes.c_double)]),
+    ("modp_gemm_config_get", _int, [_vp, _int, _i64, _i64, _i64, _i64, ctypes.POINTER(GemmConfig),
+                                    ctypes.POINTER(_int)]),
+    ("modp_gemm_ex", _int, [_vp, ctypes.POINTER(GemmConfig), _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64,
+                            _vp, _i64, _i64]),
+    ("modp_autotune_export", _int, [_vp, ctypes.c_char_p, _sz, ctypes.POINTER(_sz)]),
+    ("modp_autotune_import", _int, [_vp, ctypes.c_char_p]),
     ("modp_pluq", _int, [_vp, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64,
                          _p64, _p64, _p64, _p64, _p64]),
     ("modp_lu", _int, [_vp, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64,

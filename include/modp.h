@@ -49,9 +49,17 @@ typedef enum { MODP_F64 = 0, MODP_F32 = 1 } modp_dtype;
 typedef enum {
     MODP_FORM_AUTO = 0,      /* autotuner's choice for (shape, bits of p) */
     MODP_FORM_PLAIN_F64 = 1, /* FP64 MFMA, balanced residues, periodic reduction */
-    MODP_FORM_LIMB_I8 = 2    /* 7-bit limbs in int8, int32-accumulating MFMA,
-                                Barrett recombination */
+    MODP_FORM_LIMB_I8 = 2    /* signed base-256 digits (limbs) in int8,
+                                int32-accumulating MFMA, FP64 Barrett recombination */
 } modp_form;
+
+/* autotuner modes (modp_autotune_mode) */
+typedef enum {
+    MODP_TUNE_OFF = 0,     /* heuristics only (default) */
+    MODP_TUNE_CACHED = 1,  /* use tuned configurations present in the cache */
+    MODP_TUNE_ON_MISS = 2  /* also tune, on the call's own operands, a shape bucket
+                              missing from the cache */
+} modp_tune_mode;
 
 /* ---- version, context, errors ---------------------------------------- */
 
@@ -102,6 +110,38 @@ int modp_eye(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int6
 int modp_gemm(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, int64_t k,
               const void *A, int64_t lda, const void *B, int64_t ldb,
               void *C, int64_t ldc, int64_t p);
+
+/* ---- GEMM launch configurations and the autotuner ---------------------
+ * None of these has a counterpart in the reference (its only knob is the
+ * stripe length of stripe_mul.jl:13-27, fixed by the modulus).  A
+ * configuration is (form, output tile, K slices 1..16 run concurrently and
+ * summed mod p).  MODP_FORM_AUTO products and batched plans
+ * take their configuration from the context's tuning cache when the mode
+ * allows, else from built-in heuristics. */
+typedef struct {
+    int32_t form;    /* MODP_FORM_PLAIN_F64 or MODP_FORM_LIMB_I8 */
+    int32_t tile_m;  /* output tile: 64x64 or 128x128; the limb form also 128x64 */
+    int32_t tile_n;
+    int32_t ksplit;  /* 1..16 */
+} modp_gemm_config;
+
+int modp_autotune_mode(modp_ctx *ctx, int mode);
+/* Time every valid configuration for an m x n x k product mod p on synthetic
+ * operands, cache the fastest and return it. */
+int modp_autotune_gemm(modp_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t k, int64_t p,
+                       modp_gemm_config *best, double *best_ms);
+/* The configuration a MODP_FORM_AUTO product of this shape would run now;
+ * *tuned = 1 when it comes from the cache. */
+int modp_gemm_config_get(modp_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t k, int64_t p,
+                         modp_gemm_config *cfg, int *tuned);
+/* modp_gemm with an explicit configuration */
+int modp_gemm_ex(modp_ctx *ctx, const modp_gemm_config *cfg, int dtype, int64_t m, int64_t n, int64_t k,
+                 const void *A, int64_t lda, const void *B, int64_t ldb, void *C, int64_t ldc, int64_t p);
+/* The cache as text (one line per entry), to persist tuning across
+ * processes.  *len receives the size including the terminating NUL; buf may
+ * be NULL to query it. */
+int modp_autotune_export(modp_ctx *ctx, char *buf, size_t cap, size_t *len);
+int modp_autotune_import(modp_ctx *ctx, const char *text);
 
 /* ---- elimination ------------------------------------------------------ */
 
@@ -166,9 +206,11 @@ typedef struct {
 typedef struct modp_gemm_plan modp_gemm_plan;
 
 /* Build a plan for `count` independent products C_i = A_i*B_i mod p_i (device
- * pointers in the jobs): jobs are sorted by cost, cut into output tiles whose
- * shape the autotuner picks per job, and a persistent kernel that pulls tiles
- * from a device work queue is captured in a hipGraph. */
+ * pointers in the jobs): jobs are sorted by cost and cut into 32x32 or 64x64
+ * output tiles by a threshold that the autotuner picks per batch signature
+ * (under MODP_TUNE_ON_MISS a new signature is tuned by running the candidate
+ * plans, which rewrites the outputs), and a persistent kernel that pulls
+ * tiles from a device work queue is captured in a hipGraph. */
 int modp_gemm_batched_plan(modp_ctx *ctx, int dtype, const modp_gemm_job *jobs, int64_t count,
                            modp_gemm_plan **out);
 int modp_gemm_batched_run(modp_ctx *ctx, modp_gemm_plan *plan);

@@ -91,8 +91,39 @@ def test_extreme_residues(gffm, p, form):
     assert np.all(g.Array(C).astype(np.int64) == k % p)
 
 
+@pytest.mark.parametrize("p", [257, 65537, 16777213, 16777259])
+def test_limb_digit_count_boundaries(gffm, p):
+    # moduli just above and below 256^L: the signed base-256 digit count
+    # changes there (257 -> 2 digits, 65537 -> 3, 2^24-3 -> 3, 2^24+43 -> 4)
+    g = gffm
+    m, n, k = 65, 70, 300
+    print(f"gemm limb digit boundary p={p}", flush=True)
+    Ah = O.draw_matrix(m, k, p, seed=11)
+    Bh = O.draw_matrix(k, n, p, seed=11, offset=m * k)
+    Ah[0, :] = p - 1
+    Bh[:, 0] = p - 1
+    C = g.zeros(np.float64, m, n, p)
+    g.mul_(C, dev(g, Ah, p), dev(g, Bh, p), form=2)
+    assert np.array_equal(g.Array(C).astype(np.int64), O.gemm(Ah, Bh, p))
+
+
+def test_limb_extreme_digits_two_chunks(gffm):
+    # p = 65521: the residue 32640 is represented as 32640 - p = -32881,
+    # digits (-113, -128), the largest digit products the 2-digit form meets;
+    # K = 70,000 spans two int32-safe chunks (65,472 each)
+    g = gffm
+    p, m, n, k = 65521, 33, 17, 70000
+    print("gemm limb extreme digits p=65521 k=70000", flush=True)
+    Ah = np.full((m, k), 32640, dtype=np.int64)
+    Bh = np.full((k, n), 32640, dtype=np.int64)
+    Ah[1, :] = O.draw_matrix(1, k, p, seed=5)[0]
+    C = g.zeros(np.float64, m, n, p)
+    g.mul_(C, dev(g, Ah, p), dev(g, Bh, p), form=2)
+    assert np.array_equal(g.Array(C).astype(np.int64), O.gemm(Ah, Bh, p))
+
+
 def test_limb_k_chunking(gffm):
-    # K beyond one int32-safe chunk of the 4-limb form (33,280): chunks are
+    # K beyond one int32-safe chunk of the 4-digit form (32,704): chunks are
     # recombined mod p
     g = gffm
     p, m, n, k = 2 ** 26 - 5, 40, 24, 70000
