@@ -34,8 +34,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=8192)
-    ap.add_argument("--only", choices=["all", "gemm"], default="all",
-                    help="gemm: only the headline GEMM (for profiling runs)")
+    ap.add_argument("--only", choices=["all", "gemm", "pluq", "batched"], default="all",
+                    help="one section only, for profiling runs: the headline GEMM, the 4096^2 "
+                         "eliminations, or the batched forms")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device copy of 8*n*n bytes (calibrates HBM byte counters)")
@@ -201,6 +202,43 @@ def bench_batched_elim(g, ctx):
                 batched_pluq_gops_upper=round(ops / (ms * 1e-3) / 1e9, 2))
 
 
+def bench_pluq(g, ctx):
+    """PLUQ of 4096^2 mod 65521 (seed 2) with both forms of the trailing
+    update, and the rank-3000 product (4096x3000)(3000x4096) by PLUQ and RREF."""
+    q, nq = 65521, 4096
+    extra = {}
+    Aq = g.rand(np.float64, nq, nq, q, seed=2)
+    for name, form in (("limb", g.FORM_LIMB_I8), ("plain", g.FORM_PLAIN_F64)):
+        g.pluq(Aq, form=form)
+        ctx.synchronize()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _, _, _, _, rk = g.pluq(Aq, form=form)
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        extra[f"pluq_sec_{name}"] = round(min(ts), 5)
+        extra["pluq_rank"] = rk
+    extra["pluq_limb_vs_plain"] = round(extra["pluq_sec_plain"] / extra["pluq_sec_limb"], 3)
+    # rank-3000 product (4096x3000)(3000x4096)
+    X = g.rand(np.float64, nq, 3000, q, seed=2)
+    Y = g.rand(np.float64, 3000, nq, q, seed=2, offset=nq * 3000)
+    A3 = X * Y
+    del X, Y
+    g.pluq(A3, form=g.FORM_LIMB_I8)
+    t0 = time.perf_counter()
+    _, _, _, _, rk3 = g.pluq(A3, form=g.FORM_LIMB_I8)
+    ctx.synchronize()
+    extra.update(pluq_rank3000_sec=round(time.perf_counter() - t0, 5), pluq_rank3000_rank=rk3)
+    t0 = time.perf_counter()
+    _, _, rk3r = g.rref_full(A3, form=g.FORM_LIMB_I8)
+    ctx.synchronize()
+    extra.update(rref_rank3000_sec=round(time.perf_counter() - t0, 5))
+    del A3, Aq
+
+    return extra
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -209,6 +247,12 @@ def main():
         sys.exit(2)
     import gpufinitefieldmatrices as g
     ctx = g.context()
+    if a.only in ("pluq", "batched"):
+        # one section, for profiling; the headline GEMM is not run
+        out = bench_pluq(g, ctx) if a.only == "pluq" else {**bench_batched_gemm(g, ctx, reps=5),
+                                                            **bench_batched_elim(g, ctx)}
+        print(json.dumps({"section": a.only, **out}))
+        return
     n = a.n
     p = 2 ** 26 - 5
     ops = 2.0 * n * n * n
@@ -252,37 +296,7 @@ def main():
         extra.update(c1_plain_1024_gops=round(2.0 * 1024 ** 3 / (c1_ms * 1e-3) / 1e9, 2), c1_plain_1024_ms=round(c1_ms, 4))
         del A1, B1, C1
 
-        # PLUQ 4096^2 mod 65521 (seed 2), both forms of the trailing update
-        q, nq = 65521, 4096
-        Aq = g.rand(np.float64, nq, nq, q, seed=2)
-        for name, form in (("limb", g.FORM_LIMB_I8), ("plain", g.FORM_PLAIN_F64)):
-            g.pluq(Aq, form=form)
-            ctx.synchronize()
-            ts = []
-            for _ in range(3):
-                t0 = time.perf_counter()
-                _, _, _, _, rk = g.pluq(Aq, form=form)
-                ctx.synchronize()
-                ts.append(time.perf_counter() - t0)
-            extra[f"pluq_sec_{name}"] = round(min(ts), 5)
-            extra["pluq_rank"] = rk
-        extra["pluq_limb_vs_plain"] = round(extra["pluq_sec_plain"] / extra["pluq_sec_limb"], 3)
-        # rank-3000 product (4096x3000)(3000x4096)
-        X = g.rand(np.float64, nq, 3000, q, seed=2)
-        Y = g.rand(np.float64, 3000, nq, q, seed=2, offset=nq * 3000)
-        A3 = X * Y
-        del X, Y
-        g.pluq(A3, form=g.FORM_LIMB_I8)
-        t0 = time.perf_counter()
-        _, _, _, _, rk3 = g.pluq(A3, form=g.FORM_LIMB_I8)
-        ctx.synchronize()
-        extra.update(pluq_rank3000_sec=round(time.perf_counter() - t0, 5), pluq_rank3000_rank=rk3)
-        t0 = time.perf_counter()
-        _, _, rk3r = g.rref_full(A3, form=g.FORM_LIMB_I8)
-        ctx.synchronize()
-        extra.update(rref_rank3000_sec=round(time.perf_counter() - t0, 5))
-        del A3, Aq
-
+        extra.update(bench_pluq(g, ctx))
         extra.update(bench_batched_gemm(g, ctx, reps=5))
         extra.update(bench_batched_elim(g, ctx))
     else:
@@ -300,10 +314,20 @@ def main():
     tfile = os.path.join(REPO, "profiles", "gemm_limb_8192_traffic.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
+    # peaks measured on this device by the library's microbenchmarks
+    try:
+        measured = {"int8_mfma_tops": round(ctx.probe_peak(0) / 1e12, 1),
+                    "fp64_mfma_tflops": round(ctx.probe_peak(1) / 1e12, 2),
+                    "hbm_copy_gbs": round(ctx.probe_peak(2) / 1e9, 1)}
+    except Exception as e:  # reported, never fatal
+        measured = {"error": str(e)}
     roof = {"bound": "mfma", "kernel": "k_gemm_limb<4 digits, 64x64 tile>",
             "achieved": round(achieved_tops, 2) if achieved_tops else None,
             "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
             "frac": round(achieved_tops / INT8_PEAK_TOPS, 4) if achieved_tops else None,
+            "measured_peaks": measured,
+            "frac_of_measured": (round(achieved_tops / measured["int8_mfma_tops"], 4)
+                                 if achieved_tops and measured.get("int8_mfma_tops") else None),
             "traffic": traffic,
             "note": "achieved = 16 digit pairs x 2*8192^3 int8 ops per launch / mean launch time (HIP events); "
                     "TFLOP/s here counts int8 ops (TOP/s)"}

@@ -283,6 +283,9 @@ int tune_gemm(modp_ctx *ctx, const GemmArgs &g, GemmConfig *best, double *best_m
 TuneKey tune_key_gemm(const GemmArgs &g);
 GemmConfig gemm_lookup_config(const modp_ctx *ctx, const GemmArgs &g, int *tuned);
 int tune_export(modp_ctx *ctx, char *buf, size_t cap, size_t *len);
+
+// device peak microbenchmarks (peak.hip)
+int probe_peak(modp_ctx *ctx, int kind, double *rate);
 int tune_import(modp_ctx *ctx, const char *text);
 
 int util_mod(modp_ctx *ctx, int kind, int64_t rows, int64_t cols, void *X, int64_t ldx, int64_t N);

@@ -11,6 +11,8 @@
 // The kernel templates live here; gemm_limb_l{1,2,3,4}.hip instantiate the
 // launchers for one digit count each (parallel compilation).
 #pragma once
+#include <type_traits>
+
 #include "common.h"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -123,9 +125,11 @@ __global__ __launch_bounds__(256) void k_split_b(int64_t k, int64_t n, const TB 
 // 64 bytes with chunk q of row r stored at slot q ^ ((r >> 2) & 3), so both
 // the 16-lane row-wise fragment reads and the 4-lanes-per-row stores touch
 // every bank group once.  DB = 1: two LDS stages; stage k+1 is stored while
-// stage k feeds the MFMAs, with one barrier per stage.
-template <int BM_, int BN_, int WGM_, int WGN_, int SWZ_, int DB_ = 0> struct LimbShape {
+// stage k feeds the MFMAs, with one barrier per stage.  DB = 2: in addition
+// two register staging sets, so each global load has two stages to land.
+template <int BM_, int BN_, int WGM_, int WGN_, int SWZ_, int DB_ = 0, int OCC_ = 1> struct LimbShape {
     static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, SWZ = SWZ_, DB = DB_;
+    static constexpr int OCC = OCC_;  // waves per SIMD the register budget must allow
     static constexpr int NT = 64 * WGM * WGN;
     static constexpr int PITCH = SWZ ? 64 : 80;
 };
@@ -136,7 +140,7 @@ template <int SWZ> __device__ __forceinline__ int lds_slot(int row, int q)
 }
 
 template <int L, typename SH, typename TC, int EPI>
-__global__ __launch_bounds__(SH::NT) void k_gemm_limb(int64_t m, int64_t n, int64_t Kp,
+__global__ __launch_bounds__(SH::NT, SH::OCC) void k_gemm_limb(int64_t m, int64_t n, int64_t Kp,
                                                      const int8_t *__restrict__ Al, int64_t Mp,
                                                      const int8_t *__restrict__ Bl, int64_t Np,
                                                      TC *__restrict__ C, int64_t ldc, ModP mp, LimbW lw,
@@ -177,15 +181,16 @@ __global__ __launch_bounds__(SH::NT) void k_gemm_limb(int64_t m, int64_t n, int6
             for (int u = 0; u < NS; ++u) acc[i][j][u] = v4i{0, 0, 0, 0};
 
     constexpr int AR = (ACH + NT - 1) / NT, BR = (BCH + NT - 1) / NT;
-    v4i ra[AR], rb[BR];
-    auto load = [&](int64_t k0) {
+    constexpr int NSET = SH::DB == 2 ? 2 : 1;
+    v4i ra[NSET][AR], rb[NSET][BR];
+    auto load = [&](int set, int64_t k0) {
 #pragma unroll
         for (int i = 0; i < AR; ++i) {
             int idx = tid + NT * i;
             if (ACH % NT == 0 || idx < ACH) {
                 int s = idx / (BM * 4), rem = idx % (BM * 4);
                 int row = rem >> 2, ch = rem & 3;
-                ra[i] = *reinterpret_cast<const v4i *>(Al + s * aplane + (size_t)(r0 + row) * Kp + k0 + ch * 16);
+                ra[set][i] = *reinterpret_cast<const v4i *>(Al + s * aplane + (size_t)(r0 + row) * Kp + k0 + ch * 16);
             }
         }
 #pragma unroll
@@ -194,18 +199,18 @@ __global__ __launch_bounds__(SH::NT) void k_gemm_limb(int64_t m, int64_t n, int6
             if (BCH % NT == 0 || idx < BCH) {
                 int s = idx / (BN * 4), rem = idx % (BN * 4);
                 int col = rem >> 2, ch = rem & 3;
-                rb[i] = *reinterpret_cast<const v4i *>(Bl + s * bplane + (size_t)(c0 + col) * Kp + k0 + ch * 16);
+                rb[set][i] = *reinterpret_cast<const v4i *>(Bl + s * bplane + (size_t)(c0 + col) * Kp + k0 + ch * 16);
             }
         }
     };
-    auto stash = [&](int buf) {
+    auto stash = [&](int set, int buf) {
 #pragma unroll
         for (int i = 0; i < AR; ++i) {
             int idx = tid + NT * i;
             if (ACH % NT == 0 || idx < ACH) {
                 int s = idx / (BM * 4), rem = idx % (BM * 4);
                 int row = rem >> 2;
-                *reinterpret_cast<v4i *>(&As[buf][s][row][lds_slot<SWZ>(row, rem & 3) * 16]) = ra[i];
+                *reinterpret_cast<v4i *>(&As[buf][s][row][lds_slot<SWZ>(row, rem & 3) * 16]) = ra[set][i];
             }
         }
 #pragma unroll
@@ -214,7 +219,7 @@ __global__ __launch_bounds__(SH::NT) void k_gemm_limb(int64_t m, int64_t n, int6
             if (BCH % NT == 0 || idx < BCH) {
                 int s = idx / (BN * 4), rem = idx % (BN * 4);
                 int col = rem >> 2;
-                *reinterpret_cast<v4i *>(&Bs[buf][s][col][lds_slot<SWZ>(col, rem & 3) * 16]) = rb[i];
+                *reinterpret_cast<v4i *>(&Bs[buf][s][col][lds_slot<SWZ>(col, rem & 3) * 16]) = rb[set][i];
             }
         }
     };
@@ -250,27 +255,49 @@ __global__ __launch_bounds__(SH::NT) void k_gemm_limb(int64_t m, int64_t n, int6
     };
 
     const int64_t nk = (kend - kbeg) / LBK;
-    if constexpr (SH::DB) {
+    if constexpr (SH::DB == 2) {
+        // stage kt lives in LDS buffer kt & 1; register set (kt + 1) & 1
+        // holds stage kt + 1 and set kt & 1 receives stage kt + 2
+        load(0, kbeg);
+        if (nk > 1) load(1, kbeg + LBK);
+        stash(0, 0);
+        if (nk > 2) load(0, kbeg + 2 * LBK);
+        __syncthreads();
+        auto step = [&](int64_t kt, auto cur_c) {
+            constexpr int cur = decltype(cur_c)::value;
+            fragments(cur);
+            if (kt + 1 < nk) stash(cur ^ 1, cur ^ 1);
+            if (kt + 3 < nk) load(cur ^ 1, kbeg + (kt + 3) * LBK);
+            mfmas();
+            __syncthreads();
+        };
+        int64_t kt = 0;
+        for (; kt + 1 < nk; kt += 2) {
+            step(kt, std::integral_constant<int, 0>());
+            step(kt + 1, std::integral_constant<int, 1>());
+        }
+        if (kt < nk) step(kt, std::integral_constant<int, 0>());
+    } else if constexpr (SH::DB == 1) {
         // stage kt lives in buffer kt & 1; the registers hold stage kt + 1
-        load(kbeg);
-        stash(0);
-        if (nk > 1) load(kbeg + LBK);
+        load(0, kbeg);
+        stash(0, 0);
+        if (nk > 1) load(0, kbeg + LBK);
         __syncthreads();
         for (int64_t kt = 0; kt < nk; ++kt) {
             const int cur = (int)(kt & 1);
             fragments(cur);
             // buffer cur ^ 1 was last read before the previous barrier
-            if (kt + 1 < nk) stash(cur ^ 1);
-            if (kt + 2 < nk) load(kbeg + (kt + 2) * LBK);
+            if (kt + 1 < nk) stash(0, cur ^ 1);
+            if (kt + 2 < nk) load(0, kbeg + (kt + 2) * LBK);
             mfmas();
             __syncthreads();
         }
     } else {
-        load(kbeg);
+        load(0, kbeg);
         for (int64_t kt = 0; kt < nk; ++kt) {
-            stash(0);
+            stash(0, 0);
             __syncthreads();
-            if (kt + 1 < nk) load(kbeg + (kt + 1) * LBK);
+            if (kt + 1 < nk) load(0, kbeg + (kt + 1) * LBK);
             fragments(0);
             mfmas();
             __syncthreads();

@@ -143,6 +143,12 @@ class Context:
         self.check(self.lib.modp_event_elapsed(self.h, a, b, ctypes.byref(ms)))
         return ms.value
 
+    def probe_peak(self, kind: int) -> float:
+        """Measured peak: 0 int8 MFMA ops/s, 1 FP64 MFMA flop/s, 2 HBM bytes/s."""
+        r = ctypes.c_double()
+        self.check(self.lib.modp_probe_peak(self.h, int(kind), ctypes.byref(r)), "probe_peak")
+        return r.value
+
     # autotuner (no counterpart in the reference)
     def autotune_mode(self, mode: int):
         """TUNE_OFF (heuristics), TUNE_CACHED, or TUNE_ON_MISS."""

@@ -108,6 +108,7 @@ SIGNATURES = [
     ("modp_kernel_timing_reset", _int, [_vp]),
     ("modp_event_record", _int, [_vp, _int]),
     ("modp_event_elapsed", _int, [_vp, _int, _int, ctypes.POINTER(ctypes.c_float)]),
+    ("modp_probe_peak", _int, [_vp, _int, ctypes.POINTER(ctypes.c_double)]),
 ]
 
 _LIB = None

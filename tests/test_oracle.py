@@ -327,3 +327,47 @@ def test_is_rref_catches_planted_mistake():
     R2 = R.copy()
     R2[1, 0] = 5  # a nonzero below a pivot
     assert not O.is_rref(R2, 65521)[0]
+
+
+def _c5_instances(count=4096, seed=4):
+    """The batched-elimination workload (same construction and draw order as
+    bench.py): rows, cols ~ U[8,128], p from {2,3,7,251,65521}, every fourth
+    instance rank-deficient (a random row subset zeroed, then left-multiplied
+    by a random matrix redrawn until invertible mod p)."""
+    rng = np.random.default_rng(seed)
+    primes = [2, 3, 7, 251, 65521]
+    out = []
+    for i in range(count):
+        m, n = (int(x) for x in rng.integers(8, 129, size=2))
+        p = primes[int(rng.integers(0, 5))]
+        A = rng.integers(0, p, size=(m, n))
+        if i % 4 == 0:
+            A[rng.choice(m, size=int(rng.integers(1, m)), replace=False), :] = 0
+            while True:
+                S = rng.integers(0, p, size=(m, m))
+                if O.pluq(S, p)["rank"] == m:
+                    break
+            A = O.gemm(S, A, p)
+        out.append((np.asarray(A, dtype=np.int64), p))
+    return out
+
+
+def test_c5_src_and_optc_differences():
+    # how often the reference's pivot loop (src) and the build's Option C
+    # semantics differ on the batched-elimination workload, and that every
+    # difference is an instance where src is not a factorisation
+    diff = invalid_src = 0
+    for A, p in _c5_instances():
+        s = O.pluq(A, p, O.PLUQ_SRC)
+        c = O.pluq(A, p, O.PLUQ_OPTC)
+        same = (np.array_equal(s["U"], c["U"]) and np.array_equal(s["L"], c["L"]) and not s["L_extra"].any()
+                and s["row_swaps"] == c["row_swaps"] and s["col_swaps"] == c["col_swaps"]
+                and s["rank"] == c["rank"])
+        if same:
+            continue
+        diff += 1
+        assert O.check_pluq(A, p, c["U"], c["L"], c["row_swaps"], c["col_swaps"], c["rank"]) == []
+        if O.check_pluq(A, p, s["U"], s["L"], s["row_swaps"], s["col_swaps"], s["rank"]) != []:
+            invalid_src += 1
+    print(f"C5: src and Option C differ on {diff} of 4096 instances; src invalid on {invalid_src}")
+    assert invalid_src == diff
