@@ -653,3 +653,45 @@ int modp_probe_peak(modp_ctx *ctx, int kind, double *rate)
     if (kind < 0 || kind > 2) return set_error(ctx, MODP_ERR_ARG, "unknown peak kind %d", kind);
     return probe_peak(ctx, kind, rate);
 }
+
+int modp_trtri(modp_ctx *ctx, int dtype, int uplo, int64_t m, int64_t n, const void *A, int64_t lda, void *Ainv,
+               int64_t ldai, int64_t p)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    if (uplo != 0 && uplo != 1) return set_error(ctx, MODP_ERR_ARG, "uplo must be 0 (upper) or 1 (lower)");
+    MODP_TRY(check_mat(ctx, "A", A, m, n, lda));
+    MODP_TRY(check_mat(ctx, "Ainv", Ainv, n, m, ldai));
+    MODP_TRY(check_modulus_elim(ctx, dtype, p));
+    return tri_inverse(ctx, kind, uplo == 0, m, n, A, lda, Ainv, ldai, p);
+}
+
+int modp_inverse(modp_ctx *ctx, int form, int dtype, int64_t n, const void *A, int64_t lda, void *Ainv, int64_t ldai,
+                 int64_t p, int *invertible)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    if (!invertible) return set_error(ctx, MODP_ERR_ARG, "null output");
+    if (form != MODP_FORM_AUTO && form != MODP_FORM_PLAIN_F64 && form != MODP_FORM_LIMB_I8)
+        return set_error(ctx, MODP_ERR_ARG, "unknown form %d", form);
+    MODP_TRY(check_mat(ctx, "A", A, n, n, lda));
+    MODP_TRY(check_mat(ctx, "Ainv", Ainv, n, n, ldai));
+    MODP_TRY(check_modulus_elim(ctx, dtype, p));
+    return mat_inverse(ctx, form, kind, n, A, lda, Ainv, ldai, p, invertible);
+}
+
+int modp_gemv(modp_ctx *ctx, int dtype, int64_t m, int64_t k, const void *A, int64_t lda, const void *x, void *y,
+              int64_t p)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(check_mat(ctx, "A", A, m, k, lda));
+    if ((k > 0 && !x) || (m > 0 && !y)) return set_error(ctx, MODP_ERR_ARG, "null vector");
+    MODP_TRY(check_modulus_gemm(ctx, dtype, p));
+    if (p > ((int64_t)1 << 26))
+        return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld above 2^26", (long long)p);
+    return gemv_run(ctx, kind, m, k, A, lda, x, y, p);
+}

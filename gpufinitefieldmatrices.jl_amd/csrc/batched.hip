@@ -67,15 +67,27 @@ __global__ __launch_bounds__(256) void k_batched_gemm(const BJob *__restrict__ j
         const BItem item = items[it];
         const BJob &j = jobs[item.job];
         const int ti = item.tile / j.tiles_n, tj = item.tile % j.tiles_n;
+        // contractions that fit the exact range need no reduction at all
+        const bool exact = (int64_t)j.R >= ((int64_t)j.k + 3) / 4;
         if (j.tile == 64) {
-            gemm_f64_tile<T, T, T, 64, 64, EPI_STORE>(j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb,
-                                                      (T *)j.C, j.ldc, j.mp, j.R, (int64_t)ti * 64,
-                                                      (int64_t)tj * 64, sm64);
+            if (exact)
+                gemm_f64_tile<T, T, T, 64, 64, EPI_STORE, RED_NONE>(
+                    j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb, (T *)j.C, j.ldc, j.mp, 1,
+                    (int64_t)ti * 64, (int64_t)tj * 64, sm64);
+            else
+                gemm_f64_tile<T, T, T, 64, 64, EPI_STORE, RED_STEP>(
+                    j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb, (T *)j.C, j.ldc, j.mp, 1,
+                    (int64_t)ti * 64, (int64_t)tj * 64, sm64);
         } else {
-            gemm_f64_tile<T, T, T, 32, 32, EPI_STORE>(j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb,
-                                                      (T *)j.C, j.ldc, j.mp, j.R, (int64_t)ti * 32,
-                                                      (int64_t)tj * 32,
-                                                      *reinterpret_cast<F64Smem<32, 32> *>(&sm64));
+            F64Smem<32, 32> &sm32 = *reinterpret_cast<F64Smem<32, 32> *>(&sm64);
+            if (exact)
+                gemm_f64_tile<T, T, T, 32, 32, EPI_STORE, RED_NONE>(
+                    j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb, (T *)j.C, j.ldc, j.mp, 1,
+                    (int64_t)ti * 32, (int64_t)tj * 32, sm32);
+            else
+                gemm_f64_tile<T, T, T, 32, 32, EPI_STORE, RED_STEP>(
+                    j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb, (T *)j.C, j.ldc, j.mp, 1,
+                    (int64_t)ti * 32, (int64_t)tj * 32, sm32);
         }
     }
 }

@@ -177,7 +177,7 @@ struct TuneEntry {
     double ms = 0.0;    // measured time of the winner
 };
 
-constexpr int WS_COUNT = 12;
+constexpr int WS_COUNT = 19;
 
 struct modp_ctx {
     int device = 0;
@@ -217,7 +217,14 @@ enum {
     WS_SLICES = 8,   // K-slice partial results
     WS_TUNE_A = 9,   // synthetic operands of modp_autotune_gemm
     WS_TUNE_B = 10,
-    WS_TUNE_C = 11
+    WS_TUNE_C = 11,
+    WS_TRI_FLAG = 12,  // triangular inverse: zero-diagonal flag
+    WS_TRI_TMP = 13,   //   GEMM temporary
+    WS_TRI_T = 14,     //   int32 copy of the triangle
+    WS_TRI_X = 15,     //   int32 inverse
+    WS_TRI_X2 = 16,    //   second inverse (matrix inverse: L^{-1})
+    WS_INV_U = 17,     // matrix inverse: PLUQ factors
+    WS_INV_L = 18
 };
 
 // error plumbing: every internal function returns a modp_status
@@ -276,6 +283,8 @@ int choose_form(int64_t m, int64_t n, int64_t k, int64_t p);
 GemmConfig gemm_default_config(const modp_ctx *ctx, const GemmArgs &g, int form);
 bool gemm_config_valid(const GemmArgs &g, const GemmConfig &cfg);
 int gemm_run(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg);
+int gemv_run(modp_ctx *ctx, int kind, int64_t m, int64_t k, const void *A, int64_t lda, const void *x, void *y,
+             int64_t p);
 
 // autotuner (tune.hip)
 int gemm_auto(modp_ctx *ctx, const GemmArgs &g);
@@ -308,3 +317,9 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n,
              int64_t p, ElimResult &res);
 int elim_rref(modp_ctx *ctx, int form, int kind, int64_t m, int64_t n, const void *A, int64_t lda, void *R,
               int64_t ldr, int64_t p, ElimResult &res);
+
+// inverses (trinv.hip)
+int tri_inverse(modp_ctx *ctx, int kind, int upper, int64_t m, int64_t n, const void *A, int64_t lda, void *Ainv,
+                int64_t ldai, int64_t p);
+int mat_inverse(modp_ctx *ctx, int form, int kind, int64_t n, const void *A, int64_t lda, void *Ainv, int64_t ldai,
+                int64_t p, int *invertible);

@@ -58,6 +58,34 @@ struct PanelCtl {
 
 constexpr int PT = 512;  // threads of the panel workgroup (8 waves, 256-VGPR budget)
 
+typedef int v2i_e __attribute__((ext_vector_type(2)));
+typedef int v4i_e __attribute__((ext_vector_type(4)));
+
+// VW consecutive int32 from a VW-aligned address, one vector load
+template <int VW> __device__ __forceinline__ void load_rows(const int32_t *p, int32_t *x)
+{
+    if constexpr (VW == 4) {
+        const v4i_e t = *reinterpret_cast<const v4i_e *>(p);
+        x[0] = t[0]; x[1] = t[1]; x[2] = t[2]; x[3] = t[3];
+    } else if constexpr (VW == 2) {
+        const v2i_e t = *reinterpret_cast<const v2i_e *>(p);
+        x[0] = t[0]; x[1] = t[1];
+    } else {
+        x[0] = p[0];
+    }
+}
+
+template <int VW> __device__ __forceinline__ void store_rows(int32_t *p, const int32_t *x)
+{
+    if constexpr (VW == 4) {
+        *reinterpret_cast<v4i_e *>(p) = v4i_e{x[0], x[1], x[2], x[3]};
+    } else if constexpr (VW == 2) {
+        *reinterpret_cast<v2i_e *>(p) = v2i_e{x[0], x[1]};
+    } else {
+        p[0] = x[0];
+    }
+}
+
 // a^{-1} mod p for p < 2^24: extended Euclid whose quotients come from a
 // float reciprocal (exact operands below 2^24, estimate corrected to the
 // exact quotient); larger p use the double-division variant.
@@ -191,17 +219,34 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rows = m - kr0;
+    // Rows are read as VW-row vectors from the VW-aligned row kra <= kr0:
+    // register row q of this thread is panel row rowi(q) (negative for the
+    // sh rows above kr0, which only pad the first vector).  kra + RPT * PT >= m
+    // because the host sizes RPT from m.
+    constexpr int VW = RPT < 4 ? RPT : 4;
+    const int kra = kr0 & ~(VW - 1), sh = kr0 - kra;
+    auto rowi = [&](int q) { return VW * tid + (q % VW) + (q / VW) * (VW * PT) - sh; };
     double v[RPT][NBW];
     int pos[RPT];
     uint32_t pivmask = 0;
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        const int i = tid + PT * q;
-        pos[q] = i;
+    for (int g = 0; g < RPT / VW; ++g) {
+        const int64_t base = (int64_t)kra + VW * tid + g * (VW * PT);  // first row of the vector
 #pragma unroll
-        for (int j = 0; j < NBW; ++j)
-            v[q][j] = (i < rows && j < w) ? (double)W[(int64_t)(kr0 + i) + (int64_t)(c0 + j) * ldw] : 0.0;
+        for (int j = 0; j < NBW; ++j) {
+            int32_t x[VW];
+#pragma unroll
+            for (int e = 0; e < VW; ++e) x[e] = 0;
+            if (j < w && base < ldw) load_rows<VW>(W + base + (int64_t)(c0 + j) * ldw, x);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                const int i = rowi(g * VW + e);
+                v[g * VW + e][j] = (i >= 0 && i < rows) ? (double)x[e] : 0.0;
+            }
+        }
     }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) pos[q] = rowi(q);
     if (tid == 0) s_cnt = 0;
 
     int pc = 0, status = 0, consumed = w, par = 0;
@@ -223,18 +268,21 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             if (SMALL) v[q][jc] = dmod(v[q][jc], mp);
-            const int i = tid + PT * q;
-            if (i < rows && !((pivmask >> q) & 1u)) {
+            const int i = rowi(q);
+            if (i >= 0 && i < rows && !((pivmask >> q) & 1u)) {
                 const key_t k = K::make((uint32_t)v[q][jc], pos[q]);
                 if (k > mine) { mine = k; mq = q; }
             }
         }
+        // every lane fetches the inverse of its own candidate before the wave
+        // reduction, so the table load overlaps it
+        const uint32_t myinv = (invtab && K::val(mine) != 0) ? invtab[K::val(mine)] : 0u;
         const key_t wbest = wave_max(mine);
         if (mine == wbest && K::val(wbest) != 0) {
             // this lane holds the wave's candidate: publish row, index, inverse
             const uint32_t V = K::val(wbest);
-            s_cinv[par][wave] = invtab ? invtab[V] : modinv_fast(V, mp);
-            s_cidx[par][wave] = tid + PT * mq;
+            s_cinv[par][wave] = invtab ? myinv : modinv_fast(V, mp);
+            s_cidx[par][wave] = rowi(mq);
 #pragma unroll
             for (int j = 0; j < NBW; ++j)
                 if (j > jc) {
@@ -284,7 +332,7 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
         // positions: the winner moves to pc, the row at pc moves to Ip
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            const int i = tid + PT * q;
+            const int i = rowi(q);
             if (i == wrow) { pos[q] = pc; pivmask |= 1u << q; }
             else if (pos[q] == pc) pos[q] = Ip;
         }
@@ -299,8 +347,8 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
         }
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            const int i = tid + PT * q;
-            const double l = (i < rows && !((pivmask >> q) & 1u)) ? v[q][jc] : 0.0;
+            const int i = rowi(q);
+            const double l = (i >= 0 && i < rows && !((pivmask >> q) & 1u)) ? v[q][jc] : 0.0;
 #pragma unroll
             for (int j = 0; j < NBW; ++j)
                 if (j > jc) {
@@ -322,12 +370,34 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
     __syncthreads();
 
     // every row to its final position; pivot rows take 1 at their pivot
-    // column and the normalised U row after it.  Row gather map: positions
-    // 0..pc-1 first, in order, then every other row that moved.
+    // column and the normalised U row after it.  A vector of VW rows none of
+    // which moved is stored with one vector store per column; other rows one
+    // by one.  Row gather map: positions 0..pc-1 first, in order, then every
+    // other row that moved.
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        const int i = tid + PT * q;
-        if (i < rows) {
+    for (int g = 0; g < RPT / VW; ++g) {
+        bool still = true;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            const int q = g * VW + e, i = rowi(q);
+            still = still && i >= 0 && i < rows && pos[q] == i && !((pivmask >> q) & 1u);
+        }
+        if (still) {
+            const int64_t base = (int64_t)kr0 + rowi(g * VW);
+#pragma unroll
+            for (int j = 0; j < NBW; ++j)
+                if (j < w) {
+                    int32_t x[VW];
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) x[e] = (int32_t)(uint32_t)dmod(v[g * VW + e][j], mp);
+                    store_rows<VW>(W + base + (int64_t)(c0 + j) * ldw, x);
+                }
+            continue;
+        }
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            const int q = g * VW + e, i = rowi(q);
+            if (i < 0 || i >= rows) continue;
             const int dst = kr0 + pos[q];
             if ((pivmask >> q) & 1u) {
                 const int k = pos[q], jk = s_pcol[k];
@@ -341,6 +411,13 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
                 for (int j = 0; j < NBW; ++j)
                     if (j < w) W[dst + (int64_t)(c0 + j) * ldw] = (int32_t)(uint32_t)dmod(v[q][j], mp);
             }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int i = rowi(q);
+        if (i >= 0 && i < rows) {
+            const int dst = kr0 + pos[q];
             if (pos[q] < pc) {
                 map_dst[pos[q]] = dst;
                 map_src[pos[q]] = kr0 + i;

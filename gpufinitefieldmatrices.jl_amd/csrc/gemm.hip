@@ -24,11 +24,11 @@ namespace {
 
 // blockIdx.y selects a K slice [y*kslice, (y+1)*kslice); slice y writes its
 // residues at C + y*cslice (one slice: kslice >= k, cslice = 0).
-template <typename TA, typename TB, typename TC, int BM, int BN, int EPI>
-__global__ __launch_bounds__(256) void k_gemm_f64(int64_t m, int64_t n, int64_t k,
+template <typename TA, typename TB, typename TC, int BM, int BN, int EPI, int RED>
+__global__ __launch_bounds__(256, 2) void k_gemm_f64(int64_t m, int64_t n, int64_t k,
                                                   const TA *__restrict__ A, int64_t lda,
                                                   const TB *__restrict__ B, int64_t ldb,
-                                                  TC *__restrict__ C, int64_t ldc, ModP mp, int R,
+                                                  TC *__restrict__ C, int64_t ldc, ModP mp, int rq,
                                                   int64_t kslice, int64_t cslice,
                                                   const int *__restrict__ skip)
 {
@@ -38,8 +38,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int64_t m, int64_t n, int64_t 
     const int64_t kb = (int64_t)blockIdx.y * kslice;
     const int64_t klen = k - kb < kslice ? k - kb : kslice;
     __shared__ F64Smem<BM, BN> sm;
-    gemm_f64_tile<TA, TB, TC, BM, BN, EPI>(m, n, klen, A + kb * lda, lda, B + kb, ldb, C + blockIdx.y * cslice, ldc,
-                                           mp, R, tm * BM, tn * BN, sm);
+    gemm_f64_tile<TA, TB, TC, BM, BN, EPI, RED>(m, n, klen, A + kb * lda, lda, B + kb, ldb, C + blockIdx.y * cslice,
+                                                ldc, mp, rq, tm * BM, tn * BN, sm);
 }
 
 // C = epi(C, sum of S slice results mod p); slice s holds m x n residues at
@@ -62,63 +62,113 @@ __global__ __launch_bounds__(256) void k_slice_sum(int64_t m, int64_t n, const i
     *dst = from_res<TC>(r);
 }
 
+// y = A x mod p (A m x k column-major, x and y contiguous): one thread per
+// row, blockIdx.y a K slice.  Balanced residues; the FP64 accumulator is
+// reduced every R1 products, |a x| <= floor(p/2)^2, so it stays exact.
+// A slice writes its residues to P + y*m when S > 1, else straight to y.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv(int64_t m, int64_t k, const T *__restrict__ A, int64_t lda,
+                                              const T *__restrict__ x, T *__restrict__ y, int32_t *__restrict__ P,
+                                              int64_t kslice, int R1, ModP mp)
+{
+    __shared__ double xs[256];
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t kb = (int64_t)blockIdx.y * kslice;
+    const int64_t ke = k - kb < kslice ? k : kb + kslice;
+    const double hp = (double)mp.half;
+    double acc = 0.0;
+    int cnt = 0;
+    for (int64_t k0 = kb; k0 < ke; k0 += 256) {
+        const int64_t kk = k0 + threadIdx.x;
+        double xv = 0.0;
+        if (kk < ke) {
+            xv = (double)to_res<T>(x[kk], mp);
+            xv = xv > hp ? xv - mp.pd : xv;
+        }
+        __syncthreads();
+        xs[threadIdx.x] = xv;
+        __syncthreads();
+        const int kn = ke - k0 < 256 ? (int)(ke - k0) : 256;
+        if (r < m) {
+            const T *a = A + r + k0 * lda;
+#pragma unroll 8
+            for (int q = 0; q < kn; ++q) {
+                double av = (double)to_res<T>(a[q * lda], mp);
+                av = av > hp ? av - mp.pd : av;
+                acc = fma(av, xs[q], acc);
+                if (++cnt == R1) {
+                    cnt = 0;
+                    acc = fma(-rint(acc * mp.pinv), mp.pd, acc);
+                }
+            }
+        }
+    }
+    if (r < m) {
+        const uint32_t v = (uint32_t)dmod(acc, mp);
+        if (P) P[(int64_t)blockIdx.y * m + r] = (int32_t)v;
+        else y[r] = from_res<T>(v);
+    }
+}
+
 int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 // ---- dispatch helpers -----------------------------------------------------
 
-template <typename TA, typename TB, typename TC, int BM, int BN>
-int launch_f64_t(modp_ctx *ctx, const GemmArgs &g, int R, const Slices &sl)
+template <typename TA, typename TB, typename TC, int BM, int BN, int EPI>
+int launch_f64_e(modp_ctx *ctx, const GemmArgs &g, int R, const Slices &sl, const dim3 &grid)
 {
-    const dim3 grid(xcd_grid(((g.n + BN - 1) / BN) * ((g.m + BM - 1) / BM)), (unsigned)sl.S);
-    TimedLaunch tl(ctx, "gemm_f64");
     const TA *A = (const TA *)g.A;
     const TB *B = (const TB *)g.B;
     TC *C = (TC *)g.C;
-    switch (g.epi) {
-    case EPI_STORE:
-        k_gemm_f64<TA, TB, TC, BM, BN, EPI_STORE><<<grid, 256, 0, ctx->stream>>>(
-            g.m, g.n, g.k, A, g.lda, B, g.ldb, C, g.ldc, g.mp, R, sl.kslice, sl.cslice, g.skip);
-        break;
-    case EPI_SUB:
-        k_gemm_f64<TA, TB, TC, BM, BN, EPI_SUB><<<grid, 256, 0, ctx->stream>>>(
-            g.m, g.n, g.k, A, g.lda, B, g.ldb, C, g.ldc, g.mp, R, sl.kslice, sl.cslice, g.skip);
-        break;
-    default:
-        k_gemm_f64<TA, TB, TC, BM, BN, EPI_ADD><<<grid, 256, 0, ctx->stream>>>(
-            g.m, g.n, g.k, A, g.lda, B, g.ldb, C, g.ldc, g.mp, R, sl.kslice, sl.cslice, g.skip);
-        break;
+    int rq = 1;
+    const int red = f64_reduce_mode(R, sl.kslice < g.k ? sl.kslice : g.k, &rq);
+#define F64_LAUNCH(RED_)                                                                                      \
+    k_gemm_f64<TA, TB, TC, BM, BN, EPI, RED_><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, g.k, A, g.lda, B, g.ldb, C, \
+                                                                            g.ldc, g.mp, rq, sl.kslice,          \
+                                                                            sl.cslice, g.skip)
+    switch (red) {
+    case RED_NONE: F64_LAUNCH(RED_NONE); break;
+    case RED_STAGE: F64_LAUNCH(RED_STAGE); break;
+    case RED_2: F64_LAUNCH(RED_2); break;
+    default: F64_LAUNCH(RED_STEP); break;
     }
+#undef F64_LAUNCH
     return hip_check(ctx, hipGetLastError(), "k_gemm_f64 launch");
 }
 
-template <typename TA, typename TB, typename TC>
+template <typename TA, typename TC, int EPI>
 int launch_f64_tile(modp_ctx *ctx, const GemmArgs &g, int R, int tile, const Slices &sl)
 {
-    if (tile == 128) return launch_f64_t<TA, TB, TC, 128, 128>(ctx, g, R, sl);
-    return launch_f64_t<TA, TB, TC, 64, 64>(ctx, g, R, sl);
+    TimedLaunch tl(ctx, "gemm_f64");
+    if (tile == 128) {
+        const dim3 grid(xcd_grid(((g.n + 127) / 128) * ((g.m + 127) / 128)), (unsigned)sl.S);
+        return launch_f64_e<TA, TA, TC, 128, 128, EPI>(ctx, g, R, sl, grid);
+    }
+    const dim3 grid(xcd_grid(((g.n + 63) / 64) * ((g.m + 63) / 64)), (unsigned)sl.S);
+    return launch_f64_e<TA, TA, TC, 64, 64, EPI>(ctx, g, R, sl, grid);
 }
 
-template <typename TA, typename TB>
-int launch_f64_c(modp_ctx *ctx, const GemmArgs &g, int R, int tile, const Slices &sl)
-{
-    if (g.kc == EK_F64) return launch_f64_tile<TA, TB, double>(ctx, g, R, tile, sl);
-    if (g.kc == EK_F32) return launch_f64_tile<TA, TB, float>(ctx, g, R, tile, sl);
-    return launch_f64_tile<TA, TB, int32_t>(ctx, g, R, tile, sl);
-}
-
-template <typename TA>
-int launch_f64_b(modp_ctx *ctx, const GemmArgs &g, int R, int tile, const Slices &sl)
-{
-    if (g.kb == EK_F64) return launch_f64_c<TA, double>(ctx, g, R, tile, sl);
-    if (g.kb == EK_F32) return launch_f64_c<TA, float>(ctx, g, R, tile, sl);
-    return launch_f64_c<TA, int32_t>(ctx, g, R, tile, sl);
-}
-
+// The operand / result / epilogue combinations the library uses: user
+// products (C of the operands' type, or int32 K-slice partials), and the
+// int32 working arrays of elimination and inversion (subtracting updates,
+// products into the caller's storage type).
 int launch_f64(modp_ctx *ctx, const GemmArgs &g, int R, int tile, const Slices &sl)
 {
-    if (g.ka == EK_F64) return launch_f64_b<double>(ctx, g, R, tile, sl);
-    if (g.ka == EK_F32) return launch_f64_b<float>(ctx, g, R, tile, sl);
-    return launch_f64_b<int32_t>(ctx, g, R, tile, sl);
+    if (g.ka == g.kb) {
+        if (g.epi == EPI_STORE) {
+            if (g.ka == EK_F64 && g.kc == EK_F64) return launch_f64_tile<double, double, EPI_STORE>(ctx, g, R, tile, sl);
+            if (g.ka == EK_F64 && g.kc == EK_I32) return launch_f64_tile<double, int32_t, EPI_STORE>(ctx, g, R, tile, sl);
+            if (g.ka == EK_F32 && g.kc == EK_F32) return launch_f64_tile<float, float, EPI_STORE>(ctx, g, R, tile, sl);
+            if (g.ka == EK_F32 && g.kc == EK_I32) return launch_f64_tile<float, int32_t, EPI_STORE>(ctx, g, R, tile, sl);
+            if (g.ka == EK_I32 && g.kc == EK_I32) return launch_f64_tile<int32_t, int32_t, EPI_STORE>(ctx, g, R, tile, sl);
+            if (g.ka == EK_I32 && g.kc == EK_F64) return launch_f64_tile<int32_t, double, EPI_STORE>(ctx, g, R, tile, sl);
+            if (g.ka == EK_I32 && g.kc == EK_F32) return launch_f64_tile<int32_t, float, EPI_STORE>(ctx, g, R, tile, sl);
+        } else if (g.epi == EPI_SUB && g.ka == EK_I32 && g.kc == EK_I32) {
+            return launch_f64_tile<int32_t, int32_t, EPI_SUB>(ctx, g, R, tile, sl);
+        }
+    }
+    return set_error(ctx, MODP_ERR_DTYPE, "FP64 GEMM: unsupported operand combination (%d, %d, %d, epilogue %d)",
+                     g.ka, g.kb, g.kc, g.epi);
 }
 
 template <typename TC>
@@ -362,4 +412,46 @@ int gemm_plain_f64(modp_ctx *ctx, const GemmArgs &g)
 int gemm_limb_i8(modp_ctx *ctx, const GemmArgs &g)
 {
     return gemm_limb_cfg(ctx, g, gemm_default_config(ctx, g, MODP_FORM_LIMB_I8));
+}
+
+int gemv_run(modp_ctx *ctx, int kind, int64_t m, int64_t k, const void *A, int64_t lda, const void *x, void *y,
+             int64_t p)
+{
+    if (m <= 0) return MODP_OK;
+    const ModP mp = make_modp(p);
+    const double h = (double)(p / 2);
+    const double r1 = h > 0 ? floor((9007199254740992.0 - (double)p) / (h * h)) : 1e9;
+    const int R1 = r1 > 1e6 ? 1000000 : (int)r1;
+    if (R1 < 1) return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld too large for the FP64 form", (long long)p);
+    const int64_t rb = (m + 255) / 256;
+    // K slices until the grid holds about two workgroups per CU
+    int64_t S = (2 * (int64_t)ctx->num_cus + rb - 1) / rb;
+    if (S > (k + 511) / 512) S = (k + 511) / 512;
+    if (S < 1) S = 1;
+    const int64_t kslice = round_up((k + S - 1) / S > 0 ? (k + S - 1) / S : 1, 256);
+    S = k > 0 ? (k + kslice - 1) / kslice : 1;
+    int st = MODP_OK;
+    int32_t *P = nullptr;
+    if (S > 1) {
+        P = (int32_t *)ws_get(ctx, WS_SLICES, sizeof(int32_t) * (size_t)S * (size_t)m, &st);
+        if (!P) return st;
+    }
+    const dim3 grid((unsigned)rb, (unsigned)S);
+    {
+        TimedLaunch tl(ctx, "gemv");
+        if (kind == EK_F64)
+            k_gemv<double><<<grid, 256, 0, ctx->stream>>>(m, k, (const double *)A, lda, (const double *)x, (double *)y,
+                                                          P, kslice, R1, mp);
+        else
+            k_gemv<float><<<grid, 256, 0, ctx->stream>>>(m, k, (const float *)A, lda, (const float *)x, (float *)y, P,
+                                                         kslice, R1, mp);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    if (S == 1) return MODP_OK;
+    GemmArgs g;
+    g.m = m; g.n = 1; g.k = k;
+    g.C = y; g.ldc = m; g.kc = kind;
+    g.epi = EPI_STORE;
+    g.mp = mp;
+    return launch_slice_sum(ctx, g, EPI_STORE, P, (int)S);
 }

@@ -119,7 +119,10 @@ __global__ __launch_bounds__(256) void k_split_b(int64_t k, int64_t n, const TB 
 }
 
 // Limb-form tile shapes: a BM x BN output tile per workgroup of WGM x WGN
-// waves, each wave a (BM/WGM) x (BN/WGN) block of 16x16 MFMA blocks.
+// waves, each wave a (BM/WGM) x (BN/WGN) block of MF x MF MFMA blocks: MF = 32
+// uses v_mfma_i32_32x32x32_i8 (two k-steps per 64-deep stage), MF = 16 uses
+// v_mfma_i32_16x16x64_i8.  On gfx950 the 32x32 form sustains about 1.4x the
+// int8 rate of the 16x16 form (3.96 vs 2.8 POP/s measured).
 // LDS rows hold the 64 digits of one row (A) or column (B) for one K stage
 // as four 16-byte chunks.  SWZ = 0: rows padded to 80 bytes.  SWZ = 1: rows of
 // 64 bytes with chunk q of row r stored at slot q ^ ((r >> 2) & 3), so both
@@ -127,12 +130,15 @@ __global__ __launch_bounds__(256) void k_split_b(int64_t k, int64_t n, const TB 
 // every bank group once.  DB = 1: two LDS stages; stage k+1 is stored while
 // stage k feeds the MFMAs, with one barrier per stage.  DB = 2: in addition
 // two register staging sets, so each global load has two stages to land.
-template <int BM_, int BN_, int WGM_, int WGN_, int SWZ_, int DB_ = 0, int OCC_ = 1> struct LimbShape {
-    static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, SWZ = SWZ_, DB = DB_;
+template <int BM_, int BN_, int WGM_, int WGN_, int MF_ = 16, int SWZ_ = 1, int DB_ = 0, int OCC_ = 1>
+struct LimbShape {
+    static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, MF = MF_, SWZ = SWZ_, DB = DB_;
     static constexpr int OCC = OCC_;  // waves per SIMD the register budget must allow
     static constexpr int NT = 64 * WGM * WGN;
     static constexpr int PITCH = SWZ ? 64 : 80;
 };
+
+typedef int v16i __attribute__((ext_vector_type(16)));
 
 template <int SWZ> __device__ __forceinline__ int lds_slot(int row, int q)
 {
@@ -150,8 +156,12 @@ __global__ __launch_bounds__(SH::NT, SH::OCC) void k_gemm_limb(int64_t m, int64_
     if (skip && *skip) return;
     constexpr int BM = SH::BM, BN = SH::BN, NT = SH::NT, PITCH = SH::PITCH, SWZ = SH::SWZ;
     constexpr int NS = 2 * L - 1;
+    constexpr int MF = SH::MF;
     constexpr int WM = BM / SH::WGM, WN = BN / SH::WGN;
-    constexpr int FM = WM / 16, FN = WN / 16;
+    constexpr int FM = WM / MF, FN = WN / MF;
+    constexpr int KS = MF == 32 ? 2 : 1;     // MFMA k-steps per 64-deep stage
+    constexpr int AE = MF == 32 ? 16 : 4;    // accumulator elements per lane
+    typedef std::conditional_t<MF == 32, v16i, v4i> acc_t;
     constexpr int ACH = L * BM * (LBK / 16);  // 16-byte chunks of A per stage
     constexpr int BCH = L * BN * (LBK / 16);
     constexpr int NBUF = SH::DB ? 2 : 1;
@@ -172,13 +182,13 @@ __global__ __launch_bounds__(SH::NT, SH::OCC) void k_gemm_limb(int64_t m, int64_
     const int64_t kend = Kp - kbeg < kslice ? Kp : kbeg + kslice;
     C += blockIdx.y * cslice;
 
-    v4i acc[FM][FN][NS];
+    acc_t acc[FM][FN][NS];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int u = 0; u < NS; ++u) acc[i][j][u] = v4i{0, 0, 0, 0};
+            for (int u = 0; u < NS; ++u) acc[i][j][u] = acc_t{};
 
     constexpr int AR = (ACH + NT - 1) / NT, BR = (BCH + NT - 1) / NT;
     constexpr int NSET = SH::DB == 2 ? 2 : 1;
@@ -223,35 +233,47 @@ __global__ __launch_bounds__(SH::NT, SH::OCC) void k_gemm_limb(int64_t m, int64_
             }
         }
     };
-    v4i a[FM][L], b[FN][L];
+    // fragments: lane holds row (lane % MF) of the block and 16 digits of
+    // k-chunk q (16x16x64: q = lane / 16; 32x32x32: q = 2 ks + lane / 32)
+    v4i a[FM][KS][L], b[FN][KS][L];
     auto fragments = [&](int buf) {
-        const int q = lane >> 4;
 #pragma unroll
-        for (int i = 0; i < FM; ++i) {
-            const int row = wm * WM + i * 16 + (lane & 15);
+        for (int ks = 0; ks < KS; ++ks) {
+            const int q = MF == 32 ? ks * 2 + (lane >> 5) : (lane >> 4);
 #pragma unroll
-            for (int s = 0; s < L; ++s)
-                a[i][s] = *reinterpret_cast<const v4i *>(&As[buf][s][row][lds_slot<SWZ>(row, q) * 16]);
-        }
+            for (int i = 0; i < FM; ++i) {
+                const int row = wm * WM + i * MF + (lane & (MF - 1));
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int col = wn * WN + j * 16 + (lane & 15);
+                for (int s = 0; s < L; ++s)
+                    a[i][ks][s] = *reinterpret_cast<const v4i *>(&As[buf][s][row][lds_slot<SWZ>(row, q) * 16]);
+            }
 #pragma unroll
-            for (int t = 0; t < L; ++t)
-                b[j][t] = *reinterpret_cast<const v4i *>(&Bs[buf][t][col][lds_slot<SWZ>(col, q) * 16]);
+            for (int j = 0; j < FN; ++j) {
+                const int col = wn * WN + j * MF + (lane & (MF - 1));
+#pragma unroll
+                for (int t = 0; t < L; ++t)
+                    b[j][ks][t] = *reinterpret_cast<const v4i *>(&Bs[buf][t][col][lds_slot<SWZ>(col, q) * 16]);
+            }
         }
     };
     auto mfmas = [&]() {
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+        for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-            for (int j = 0; j < FN; ++j)
+            for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int s = 0; s < L; ++s)
+                for (int j = 0; j < FN; ++j)
 #pragma unroll
-                    for (int t = 0; t < L; ++t)
-                        acc[i][j][s + t] =
-                            __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i][s], b[j][t], acc[i][j][s + t], 0, 0, 0);
+                    for (int s = 0; s < L; ++s)
+#pragma unroll
+                        for (int t = 0; t < L; ++t) {
+                            if constexpr (MF == 32)
+                                acc[i][j][s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i][ks][s], b[j][ks][t],
+                                                                                         acc[i][j][s + t], 0, 0, 0);
+                            else
+                                acc[i][j][s + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i][ks][s], b[j][ks][t],
+                                                                                         acc[i][j][s + t], 0, 0, 0);
+                        }
     };
 
     const int64_t nk = (kend - kbeg) / LBK;
@@ -304,15 +326,19 @@ __global__ __launch_bounds__(SH::NT, SH::OCC) void k_gemm_limb(int64_t m, int64_
         }
     }
 
+    // accumulator layouts: 16x16 blocks hold column lane % 16, rows
+    // 4 (lane / 16) + e; 32x32 blocks hold column lane % 32, rows
+    // 8 (e / 4) + 4 (lane / 32) + e % 4
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-            int64_t gc = c0 + wn * WN + j * 16 + (lane & 15);
+            int64_t gc = c0 + wn * WN + j * MF + (lane & (MF - 1));
             if (gc >= n) continue;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                int64_t gr = r0 + wm * WM + i * 16 + 4 * (lane >> 4) + e;
+            for (int e = 0; e < AE; ++e) {
+                const int rr = MF == 32 ? 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3) : 4 * (lane >> 4) + e;
+                int64_t gr = r0 + wm * WM + i * MF + rr;
                 if (gr >= m) continue;
                 double tot = dmod((double)acc[i][j][0][e], mp);
 #pragma unroll
@@ -330,7 +356,8 @@ __global__ __launch_bounds__(SH::NT, SH::OCC) void k_gemm_limb(int64_t m, int64_
 }
 
 
-// LDS layout used by the library's limb kernels
+// MFMA block size and LDS layout used by the library's limb kernels
+constexpr int LIMB_MF = 16;
 constexpr int LIMB_SWZ = 1;
 
 template <int L, typename SH, typename TC>
@@ -360,10 +387,10 @@ int limb_by_shape(modp_ctx *ctx, const GemmArgs &g, int epi, const int8_t *Al, i
                   int64_t Np, int64_t Kp, const LimbW &lw, int tile_m, int tile_n, const Slices &sl)
 {
     if (L <= 2 && tile_m == 128 && tile_n == 128)
-        return limb_core<L, LimbShape<128, 128, 2, 2, LIMB_SWZ>, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, sl);
+        return limb_core<L, LimbShape<128, 128, 2, 2, LIMB_MF, LIMB_SWZ>, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, sl);
     if (tile_m == 128 && tile_n == 64)
-        return limb_core<L, LimbShape<128, 64, 4, 2, LIMB_SWZ>, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, sl);
-    return limb_core<L, LimbShape<64, 64, 2, 2, LIMB_SWZ>, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, sl);
+        return limb_core<L, LimbShape<128, 64, 4, 2, LIMB_MF, LIMB_SWZ>, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, sl);
+    return limb_core<L, LimbShape<64, 64, 2, 2, LIMB_MF, LIMB_SWZ>, TC>(ctx, g, epi, Al, Mp, Bl, Np, Kp, lw, sl);
 }
 
 }  // namespace

@@ -66,6 +66,15 @@ class CuModMatrixModulusNotPrimeException(Exception):
     pass
 
 
+class MatrixNotInvertibleException(Exception):
+    """MatrixNotInvertibleException (raised by inverse, CuModMatrix.jl:485)."""
+
+
+class InverseNotDefinedException(Exception):
+    """InverseNotDefinedException (CuModMatrix.jl:29; raised by
+    lower_triangular_inverse_no_copy, triangular_inverse_no_copy.jl:463)."""
+
+
 class ArgumentError(ValueError):
     pass
 
@@ -367,6 +376,14 @@ def Array(A: CuModArray) -> np.ndarray:
     return np.array(full[:A.size[0], :A.size[1]], order="F")
 
 
+def CuModVector(x, N: int, mod: bool = True, elem_type=DEFAULT_TYPE, ctx: Optional[Context] = None) -> CuModArray:
+    """A length-n vector mod N, held as an n x 1 CuModArray (the column of a
+    CuModVector, CuModMatrix.jl:42-122); `mul_(z, A, x)` with such vectors is
+    the matrix-vector product of stripe_mul!(z, A, x) (stripe_mul.jl:82-168)."""
+    v = np.asarray(x).reshape(-1, 1)
+    return CuModMatrix(v, N, mod=mod, elem_type=elem_type, ctx=ctx)
+
+
 def zeros(T, rows_: int, cols_: int, N: int, new_size=None, ctx: Optional[Context] = None) -> CuModArray:
     """zeros(T, rows, cols, N) (CuModMatrix.jl:530-534)."""
     M = _new(rows_, cols_, N, T, ctx)
@@ -444,6 +461,10 @@ def _gemm_raw(C: CuModArray, A: CuModArray, B: CuModArray, N: int, form: int = F
     m, k = A.size
     n = B.size[1]
     ctx = C.ctx
+    if n == 1 and form == FORM_AUTO and config is None:
+        # matrix-vector product: one streaming pass over A
+        ctx.check(ctx.lib.modp_gemv(ctx.h, C.dtype_code, m, k, A.ptr, A.ld, B.ptr, C.ptr, int(N)), "gemv")
+        return
     if config is not None:
         cfg = _lib.GemmConfig(**dict(config))
         ctx.check(ctx.lib.modp_gemm_ex(ctx.h, ctypes.byref(cfg), C.dtype_code, m, n, k, A.ptr, A.ld, B.ptr, B.ld,
@@ -575,6 +596,65 @@ def pluq(A: CuModArray, form: int = FORM_AUTO):
                                 rs.ctypes.data_as(p64), ctypes.byref(nr), cs.ctypes.data_as(p64),
                                 ctypes.byref(nc), ctypes.byref(rk)), "pluq")
     return U, L, _stack(rs, nr.value), _stack(cs, nc.value), rk.value
+
+
+def _trtri(A: CuModArray, uplo: int) -> CuModArray:
+    m, n = A.size
+    Ainv = zeros(A.elem_type, n, m, A.N, ctx=A.ctx)
+    ctx = A.ctx
+    st = ctx.lib.modp_trtri(ctx.h, A.dtype_code, uplo, m, n, A.ptr, A.ld, Ainv.ptr, Ainv.ld, A.N)
+    if st == _lib.MODP_ERR_DIM:
+        raise InverseNotDefinedException(ctx.lib.modp_last_error(ctx.h).decode(errors="replace"))
+    if st == _lib.MODP_ERR_NOT_INVERTIBLE:
+        raise MatrixNotInvertibleException(ctx.lib.modp_last_error(ctx.h).decode(errors="replace"))
+    ctx.check(st, "triangular inverse")
+    return Ainv
+
+
+def upper_triangular_inverse_no_copy(A: CuModArray, debug: bool = False) -> CuModArray:
+    """upper_triangular_inverse_no_copy(A) (triangular_inverse_no_copy.jl:197-229):
+    the (right) inverse of an upper triangular matrix, size (cols, rows)."""
+    return _trtri(A, 0)
+
+
+def lower_triangular_inverse_no_copy(A: CuModArray, debug: bool = False) -> CuModArray:
+    """lower_triangular_inverse_no_copy(A) (triangular_inverse_no_copy.jl:450-478);
+    a tall lower-triangular matrix raises InverseNotDefinedException."""
+    return _trtri(A, 1)
+
+
+def _inverse(A: CuModArray, form: int):
+    m, n = A.size
+    if m != n:
+        raise CuModMatrixNotSquareException("Matrix must be square to be inverted")
+    Ainv = zeros(A.elem_type, n, n, A.N, ctx=A.ctx)
+    ok = ctypes.c_int()
+    ctx = A.ctx
+    ctx.check(ctx.lib.modp_inverse(ctx.h, form, A.dtype_code, n, A.ptr, A.ld, Ainv.ptr, Ainv.ld, A.N,
+                                   ctypes.byref(ok)), "inverse")
+    return bool(ok.value), Ainv
+
+
+def inverse(A: CuModArray, form: int = FORM_AUTO) -> CuModArray:
+    """inverse(A) (CuModMatrix.jl:480-502): A^{-1} mod N from PLUQ and the two
+    triangular inverses; raises MatrixNotInvertibleException when the mod-N
+    rank is deficient."""
+    ok, Ainv = _inverse(A, form)
+    if not ok:
+        raise MatrixNotInvertibleException(f"Matrix is not invertible mod {A.N}")
+    return Ainv
+
+
+def is_invertible_with_inverse(A: CuModArray, debug: bool = False, form: int = FORM_AUTO):
+    """is_invertible_with_inverse(A) (CuModMatrix.jl:351-411): (true, A^{-1})
+    or (false, None)."""
+    ok, Ainv = _inverse(A, form)
+    return (True, Ainv) if ok else (False, None)
+
+
+def is_invertible(A: CuModArray, form: int = FORM_AUTO) -> bool:
+    """is_invertible(A) (CuModMatrix.jl:460-465): full rank mod N, from PLUQ."""
+    return pluq(A, form=form)[4] == min(A.size)
 
 
 def pluq_gpu_kernel(A: CuModArray, debug: bool = False, form: int = FORM_AUTO):

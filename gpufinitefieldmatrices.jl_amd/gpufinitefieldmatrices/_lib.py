@@ -82,6 +82,7 @@ SIGNATURES = [
     ("modp_rand", _int, [_vp, _int, _i64, _i64, _vp, _i64, _i64, _u64, _u64]),
     ("modp_eye", _int, [_vp, _int, _i64, _i64, _vp, _i64]),
     ("modp_gemm", _int, [_vp, _int, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64]),
+    ("modp_gemv", _int, [_vp, _int, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
     ("modp_autotune_mode", _int, [_vp, _int]),
     ("modp_autotune_gemm", _int, [_vp, _int, _i64, _i64, _i64, _i64, ctypes.POINTER(GemmConfig),
                                   ctypes.POINTER(ctypes.c_double)]),
@@ -96,6 +97,8 @@ SIGNATURES = [
     ("modp_lu", _int, [_vp, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64,
                        _p64, _p64, _p64]),
     ("modp_rref", _int, [_vp, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _p64, _p64]),
+    ("modp_trtri", _int, [_vp, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _i64]),
+    ("modp_inverse", _int, [_vp, _int, _int, _i64, _vp, _i64, _vp, _i64, _i64, ctypes.POINTER(_int)]),
     ("modp_apply_row_perm", _int, [_vp, _int, _i64, _i64, _vp, _i64, _p64, _i64, _int]),
     ("modp_apply_col_perm", _int, [_vp, _int, _i64, _i64, _vp, _i64, _p64, _i64, _int]),
     ("modp_gemm_batched_plan", _int, [_vp, _int, ctypes.POINTER(GemmJob), _i64, ctypes.POINTER(_vp)]),

@@ -111,6 +111,13 @@ int modp_gemm(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, int64_t 
               const void *A, int64_t lda, const void *B, int64_t ldb,
               void *C, int64_t ldc, int64_t p);
 
+/* y = A*x mod p: A m x k (column-major, leading dimension lda), x (k) and y
+ * (m) contiguous device vectors; entries reduced mod p on load.  Exact for
+ * 2 <= p <= 2^26.  Replaces stripe_mul!(z,A,x) (stripe_mul.jl:82-168) and
+ * mul!(z,A,x) (CuModMatrix.jl:816-836). */
+int modp_gemv(modp_ctx *ctx, int dtype, int64_t m, int64_t k, const void *A, int64_t lda, const void *x, void *y,
+              int64_t p);
+
 /* ---- GEMM launch configurations and the autotuner ---------------------
  * None of these has a counterpart in the reference (its only knob is the
  * stripe length of stripe_mul.jl:13-27, fixed by the modulus).  A
@@ -191,6 +198,27 @@ int modp_apply_row_perm(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, vo
 /* Same for columns (apply_col_perm!, :11-18; apply_col_inv_perm!, :20-27). */
 int modp_apply_col_perm(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx,
                         const int64_t *stack, int64_t nswaps, int inverse);
+
+/* ---- inverses ------------------------------------------------------------ */
+
+/* Inverse of a triangular matrix mod p.  uplo 0: A (m x n) is upper
+ * triangular, 1: lower triangular; entries outside the triangle are ignored.
+ * Ainv (n x m) receives the inverse of the leading k x k block (k = min(m,n))
+ * in its leading block and zeros elsewhere, so A*Ainv = I for square and
+ * wide matrices.  A zero diagonal entry gives MODP_ERR_NOT_INVERTIBLE; a tall
+ * lower-triangular A gives MODP_ERR_DIM (InverseNotDefinedException).
+ * Replaces upper_triangular_inverse_no_copy / lower_triangular_inverse_no_copy
+ * (src/CuModMatrix/triangular/triangular_inverse_no_copy.jl:197-229,450-478). */
+int modp_trtri(modp_ctx *ctx, int dtype, int uplo, int64_t m, int64_t n, const void *A, int64_t lda, void *Ainv,
+               int64_t ldai, int64_t p);
+
+/* Ainv = A^{-1} mod p for square A (n x n) through PLUQ:
+ * A^{-1} = Q U^{-1} L^{-1} P.  *invertible = 0 (and Ainv untouched) when the
+ * mod-p rank is below n.  Replaces inverse / is_invertible_with_inverse /
+ * is_invertible (src/CuModMatrix/CuModMatrix.jl:340-502); the rank test is
+ * the mod-p rank, where the reference takes a floating-point rank of U. */
+int modp_inverse(modp_ctx *ctx, int form, int dtype, int64_t n, const void *A, int64_t lda, void *Ainv, int64_t ldai,
+                 int64_t p, int *invertible);
 
 /* ---- batched multi-modular forms (no counterpart in the reference; the
  *      nearest precedents are gemv_batched!, KaratsubaMatrix.jl:280, and the

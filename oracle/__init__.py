@@ -160,6 +160,20 @@ def rref(A, p: int) -> dict:
     return dict(R=R, pivcols=piv[:r].copy(), rank=r)
 
 
+def inverse(A, p: int):
+    """A^{-1} mod p by Gauss-Jordan on [A | I] (the reduced form of an
+    invertible A's augmented matrix is [I | A^{-1}]); None when A is singular
+    mod p.  For a triangular A this is also its triangular inverse."""
+    A = np.asarray(A, dtype=np.int64) % p
+    n = A.shape[0]
+    if n == 0:
+        return np.zeros((0, 0), dtype=np.int64)
+    r = rref(np.hstack([A, np.eye(n, dtype=np.int64)]), p)
+    if r["rank"] < n or int(r["pivcols"][n - 1]) != n:
+        return None
+    return r["R"][:, n:].copy()
+
+
 def is_prime(n: int) -> bool:
     return bool(lib().oracle_is_prime(n))
 

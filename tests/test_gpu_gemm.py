@@ -213,3 +213,30 @@ def test_config2_8192_freivalds_and_forms_agree(gffm, p):
     # a block against the oracle exactly
     blk = O.gemm(Ah[:256], Bh[:, :256], p)
     assert np.array_equal(Ch[:256, :256], blk)
+
+
+@pytest.mark.parametrize("p", [2, 127, 65521, 8388593, 2 ** 26 - 5])
+@pytest.mark.parametrize("shape", [(1, 1), (100, 257), (257, 100), (5000, 3000), (64, 70000)], ids=str)
+def test_gemv_matches_oracle(gffm, shape, p):
+    # z = A x mod p through the streaming matrix-vector kernel (K slices
+    # summed mod p for the long shapes); extreme residues in one row
+    g = gffm
+    m, k = shape
+    print(f"gemv shape={shape} p={p}", flush=True)
+    Ah = O.draw_matrix(m, k, p, seed=31)
+    xh = O.draw_matrix(k, 1, p, seed=31, offset=m * k)
+    Ah[0, :] = p - 1
+    xh[: k // 2, 0] = p - 1
+    z = g.zeros(np.float64, m, 1, p)
+    g.mul_(z, dev(g, Ah, p), g.CuModVector(xh[:, 0], p, elem_type=np.float64))
+    assert np.array_equal(g.Array(z).astype(np.int64), O.gemm(Ah, xh, p))
+
+
+def test_gemv_float32_and_operator(gffm):
+    g = gffm
+    p = 2039
+    Ah = O.draw_matrix(300, 400, p, seed=32)
+    xh = O.draw_matrix(400, 1, p, seed=33)
+    A = dev(g, Ah, p, np.float32)
+    x = g.CuModVector(xh[:, 0], p, elem_type=np.float32)
+    assert np.array_equal(g.Array(A * x).astype(np.int64), O.gemm(Ah, xh, p))

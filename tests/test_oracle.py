@@ -371,3 +371,34 @@ def test_c5_src_and_optc_differences():
             invalid_src += 1
     print(f"C5: src and Option C differ on {diff} of 4096 instances; src invalid on {invalid_src}")
     assert invalid_src == diff
+
+
+@pytest.mark.parametrize("p", [2, 3, 7, 65521, 2 ** 26 - 5])
+def test_oracle_inverse_is_an_inverse(p):
+    # A * inv(A) = I by big-integer products; singular inputs give None
+    for seed in range(6):
+        n = 1 + seed * 7
+        A = O.draw_matrix(n, n, p, seed)
+        inv = O.inverse(A, p)
+        rank = O.rref(A, p)["rank"]
+        if rank < n:
+            assert inv is None
+            continue
+        eye = np.eye(n, dtype=np.int64)
+        assert (np.asarray(O.matmul_bigint(A, inv, p), dtype=np.int64) == eye).all()
+        assert (np.asarray(O.matmul_bigint(inv, A, p), dtype=np.int64) == eye).all()
+    S = O.draw_matrix(6, 6, p, 99)
+    S[:, 5] = S[:, 0]
+    assert O.inverse(S, p) is None
+
+
+def test_oracle_inverse_reference_fixtures():
+    # triangular_test.jl:26-31 and :74-79 (2x2 mod 7), and the identity
+    U = np.array([[1, 3], [0, 2]])
+    L = np.array([[1, 0], [3, 2]])
+    for T in (U, L):
+        inv = O.inverse(T, 7)
+        assert ((T @ inv) % 7 == np.eye(2, dtype=np.int64)).all()
+    assert (O.inverse(np.eye(50, dtype=np.int64), 2) == np.eye(50, dtype=np.int64)).all()
+    # the upper inverse is upper triangular, the lower one lower triangular
+    assert O.inverse(U, 7)[1, 0] == 0 and O.inverse(L, 7)[0, 1] == 0
