@@ -234,7 +234,16 @@ def bench_pluq(g, ctx):
     _, _, rk3r = g.rref_full(A3, form=g.FORM_LIMB_I8)
     ctx.synchronize()
     extra.update(rref_rank3000_sec=round(time.perf_counter() - t0, 5))
-    del A3, Aq
+    del A3
+    # A^{-1} of the full-rank 4096^2 matrix (PLUQ, two triangular inverses,
+    # permutations, one product)
+    g.inverse(Aq)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    g.inverse(Aq)
+    ctx.synchronize()
+    extra.update(inverse_4096_sec=round(time.perf_counter() - t0, 5))
+    del Aq
 
     return extra
 
@@ -295,6 +304,22 @@ def main():
         c1_ms, _, _, _ = time_gemm(g, ctx, A1, B1, C1, g.FORM_PLAIN_F64, 20, 3, "gemm_f64")
         extra.update(c1_plain_1024_gops=round(2.0 * 1024 ** 3 / (c1_ms * 1e-3) / 1e9, 2), c1_plain_1024_ms=round(c1_ms, 4))
         del A1, B1, C1
+
+        # matrix-vector product 8192 x 8192 mod 2^26-5: one HBM pass over A
+        Av = g.rand(np.float64, n, n, p, seed=1)
+        xv = g.rand(np.float64, n, 1, p, seed=1, offset=n * n)
+        zv = g.zeros(np.float64, n, 1, p)
+        for _ in range(3):
+            g.mul_(zv, Av, xv)
+        ctx.synchronize()
+        ctx.event_record(4)
+        for _ in range(10):
+            g.mul_(zv, Av, xv)
+        ctx.event_record(5)
+        gemv_ms = ctx.event_elapsed(4, 5) / 10
+        extra.update(gemv_8192_ms=round(gemv_ms, 4),
+                     gemv_8192_hbm_gbs=round(8.0 * n * n / (gemv_ms * 1e-3) / 1e9, 1))
+        del Av, xv, zv
 
         extra.update(bench_pluq(g, ctx))
         extra.update(bench_batched_gemm(g, ctx, reps=5))
