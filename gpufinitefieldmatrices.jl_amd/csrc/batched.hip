@@ -34,12 +34,18 @@ struct BItem {
     int tile;  // tile index within the job: ti = tile / tiles_n, tj = tile % tiles_n
 };
 
+// Work items are grouped in classes (tile 64 or 32) x (reduction needed or
+// not); each class is served by its own persistent kernel, which carries
+// only that class's tile code (fewer registers, more resident waves), and
+// pulls items from its own counter.
+constexpr int NCLASS = 4;
 struct modp_gemm_plan {
     int kind = EK_F64;
     int64_t count = 0, items = 0;
+    int64_t cls_off[NCLASS + 1] = {0};  // item ranges of the classes
     BJob *djobs = nullptr;
     BItem *ditems = nullptr;
-    int *dcounter = nullptr;
+    int *dcounter = nullptr;  // one counter per class, 64 bytes apart
     int grid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -49,14 +55,15 @@ namespace {
 
 __global__ void k_queue_reset(int *counter)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+    if (threadIdx.x < NCLASS && blockIdx.x == 0) counter[threadIdx.x * 16] = 0;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_batched_gemm(const BJob *__restrict__ jobs, const BItem *__restrict__ items,
-                                                      int64_t nitems, int *__restrict__ counter)
+template <typename T, int TILE, int RED>
+__global__ __launch_bounds__(256, 2) void k_batched_gemm(const BJob *__restrict__ jobs,
+                                                         const BItem *__restrict__ items, int64_t nitems,
+                                                         int *__restrict__ counter)
 {
-    __shared__ F64Smem<64, 64> sm64;
+    __shared__ F64Smem<TILE, TILE> sm;
     __shared__ int s_item;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
@@ -67,28 +74,28 @@ __global__ __launch_bounds__(256) void k_batched_gemm(const BJob *__restrict__ j
         const BItem item = items[it];
         const BJob &j = jobs[item.job];
         const int ti = item.tile / j.tiles_n, tj = item.tile % j.tiles_n;
-        // contractions that fit the exact range need no reduction at all
-        const bool exact = (int64_t)j.R >= ((int64_t)j.k + 3) / 4;
-        if (j.tile == 64) {
-            if (exact)
-                gemm_f64_tile<T, T, T, 64, 64, EPI_STORE, RED_NONE>(
-                    j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb, (T *)j.C, j.ldc, j.mp, 1,
-                    (int64_t)ti * 64, (int64_t)tj * 64, sm64);
-            else
-                gemm_f64_tile<T, T, T, 64, 64, EPI_STORE, RED_STEP>(
-                    j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb, (T *)j.C, j.ldc, j.mp, 1,
-                    (int64_t)ti * 64, (int64_t)tj * 64, sm64);
-        } else {
-            F64Smem<32, 32> &sm32 = *reinterpret_cast<F64Smem<32, 32> *>(&sm64);
-            if (exact)
-                gemm_f64_tile<T, T, T, 32, 32, EPI_STORE, RED_NONE>(
-                    j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb, (T *)j.C, j.ldc, j.mp, 1,
-                    (int64_t)ti * 32, (int64_t)tj * 32, sm32);
-            else
-                gemm_f64_tile<T, T, T, 32, 32, EPI_STORE, RED_STEP>(
-                    j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B, j.ldb, (T *)j.C, j.ldc, j.mp, 1,
-                    (int64_t)ti * 32, (int64_t)tj * 32, sm32);
-        }
+        gemm_f64_tile<T, T, T, TILE, TILE, EPI_STORE, RED>(j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B,
+                                                           j.ldb, (T *)j.C, j.ldc, j.mp, 1, (int64_t)ti * TILE,
+                                                           (int64_t)tj * TILE, sm);
+    }
+}
+
+template <typename T>
+void launch_class(int cls, const modp_gemm_plan *pl, hipStream_t st)
+{
+    const int64_t n = pl->cls_off[cls + 1] - pl->cls_off[cls];
+    if (n <= 0) return;
+    const BItem *it = pl->ditems + pl->cls_off[cls];
+    int *ctr = pl->dcounter + cls * 16;
+    // resident workgroups per CU by the kernels' register use: 3 for 64x64
+    // tiles, 5 for 32x32
+    const int64_t cap = (int64_t)pl->grid * (cls < 2 ? 3 : 5);
+    const unsigned g = (unsigned)(n < cap ? n : cap);
+    switch (cls) {
+    case 0: k_batched_gemm<T, 64, RED_NONE><<<g, 256, 0, st>>>(pl->djobs, it, n, ctr); break;
+    case 1: k_batched_gemm<T, 64, RED_STEP><<<g, 256, 0, st>>>(pl->djobs, it, n, ctr); break;
+    case 2: k_batched_gemm<T, 32, RED_NONE><<<g, 256, 0, st>>>(pl->djobs, it, n, ctr); break;
+    default: k_batched_gemm<T, 32, RED_STEP><<<g, 256, 0, st>>>(pl->djobs, it, n, ctr); break;
     }
 }
 
@@ -218,16 +225,24 @@ int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count
         d.tile = pick_tile(d.m, d.n, thr);
         d.tiles_n = (d.n + d.tile - 1) / d.tile;
     }
-    // most expensive tiles first
+    // most expensive tiles first, within each class
     auto cost = [&](int64_t i) { return (int64_t)hj[i].tile * hj[i].tile * hj[i].k; };
     std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
+    auto cls_of = [&](const BJob &d) {
+        const bool exact = (int64_t)d.R >= ((int64_t)d.k + 3) / 4;
+        return (d.tile == 64 ? 0 : 2) + (exact ? 0 : 1);
+    };
     std::vector<BItem> items;
-    for (int64_t oi : order) {
-        const BJob &d = hj[oi];
-        if (d.m <= 0 || d.n <= 0) continue;
-        const int tiles = ((d.m + d.tile - 1) / d.tile) * d.tiles_n;
-        for (int t = 0; t < tiles; ++t) items.push_back(BItem{(int)oi, t});
+    for (int c = 0; c < NCLASS; ++c) {
+        pl->cls_off[c] = (int64_t)items.size();
+        for (int64_t oi : order) {
+            const BJob &d = hj[oi];
+            if (d.m <= 0 || d.n <= 0 || cls_of(d) != c) continue;
+            const int tiles = ((d.m + d.tile - 1) / d.tile) * d.tiles_n;
+            for (int t = 0; t < tiles; ++t) items.push_back(BItem{(int)oi, t});
+        }
     }
+    pl->cls_off[NCLASS] = (int64_t)items.size();
     pl->items = (int64_t)items.size();
     hipError_t e = hipMalloc(&pl->djobs, sizeof(BJob) * (count > 0 ? count : 1));
     if (e == hipSuccess) e = hipMalloc(&pl->ditems, sizeof(BItem) * (items.empty() ? 1 : items.size()));
@@ -242,18 +257,16 @@ int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count
         batched_gemm_destroy(ctx, pl);
         return st;
     }
-    // persistent grid: two workgroups per CU (LDS ~ 17 KB each, 256 threads)
-    int64_t g = (int64_t)ctx->num_cus * 4;
-    if (g > pl->items) g = pl->items;
-    pl->grid = (int)(g > 0 ? g : 1);
-    // capture [reset, persistent kernel] in a graph
+    // persistent grids are a multiple of the CU count (launch_class)
+    pl->grid = ctx->num_cus > 0 ? ctx->num_cus : 1;
+    // capture [counter reset, one persistent kernel per class] in a graph
     e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         k_queue_reset<<<1, 64, 0, ctx->stream>>>(pl->dcounter);
-        if (kind == EK_F64)
-            k_batched_gemm<double><<<pl->grid, 256, 0, ctx->stream>>>(pl->djobs, pl->ditems, pl->items, pl->dcounter);
-        else
-            k_batched_gemm<float><<<pl->grid, 256, 0, ctx->stream>>>(pl->djobs, pl->ditems, pl->items, pl->dcounter);
+        for (int c = 0; c < NCLASS; ++c) {
+            if (kind == EK_F64) launch_class<double>(c, pl, ctx->stream);
+            else launch_class<float>(c, pl, ctx->stream);
+        }
         e = hipStreamEndCapture(ctx->stream, &pl->graph);
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0);
