@@ -695,3 +695,22 @@ int modp_gemv(modp_ctx *ctx, int dtype, int64_t m, int64_t k, const void *A, int
         return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld above 2^26", (long long)p);
     return gemv_run(ctx, kind, m, k, A, lda, x, y, p);
 }
+
+int modp_crt2(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, const void *X1, int64_t ld1, int64_t p1,
+              const void *X2, int64_t ld2, int64_t p2, void *Y, int64_t ldy)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(check_mat(ctx, "X1", X1, rows, cols, ld1));
+    MODP_TRY(check_mat(ctx, "X2", X2, rows, cols, ld2));
+    MODP_TRY(check_mat(ctx, "Y", Y, rows, cols, ldy));
+    const int64_t lim26 = (int64_t)1 << 26;
+    if (p1 < 2 || p2 < 2 || p1 > lim26 || p2 > lim26)
+        return set_error(ctx, MODP_ERR_MODULUS_RANGE, "moduli %lld, %lld outside 2..2^26", (long long)p1, (long long)p2);
+    const int64_t lim = dtype == MODP_F64 ? ((int64_t)1 << 53) : ((int64_t)1 << 24);
+    if (p1 * p2 > lim)
+        return set_error(ctx, MODP_ERR_MODULUS_RANGE, "p1*p2 = %lld is not exact in the storage type",
+                         (long long)(p1 * p2));
+    return util_crt2(ctx, kind, rows, cols, X1, ld1, p1, X2, ld2, p2, Y, ldy);
+}

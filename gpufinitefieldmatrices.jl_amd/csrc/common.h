@@ -36,10 +36,13 @@ static inline ModP make_modp(int64_t p)
     return m;
 }
 
-// x in (-2^53, 2^53), integral -> x mod p in [0,p)
+// x in (-2^53, 2^53), integral -> x mod p in [0,p).  q = rint(x/p) is off
+// by at most one (fl(x * pinv) is within 2^53 * 2^-52 / p <= 2/3 of x/p), so
+// r = x - q p is exact (|r| < 2^53) and |r| < 1.5p + 1; three fix-ups bring it
+// into [0,p).  (Round-to-nearest runs at twice the rate of floor here.)
 __device__ __forceinline__ double dmod(double x, const ModP &mp)
 {
-    double q = floor(x * mp.pinv);
+    double q = rint(x * mp.pinv);
     double r = fma(-q, mp.pd, x);
     r = r < 0.0 ? r + mp.pd : r;
     r = r >= mp.pd ? r - mp.pd : r;
@@ -305,6 +308,8 @@ int util_convert(modp_ctx *ctx, int kin, int64_t rows, int64_t cols, const void 
                  int kout, void *Y, int64_t ldy, const ModP &mp);
 int util_perm(modp_ctx *ctx, int kind, int rowwise, int64_t rows, int64_t cols, void *X, int64_t ldx,
               const int64_t *stack_host, int64_t nswaps, int inverse);
+int util_crt2(modp_ctx *ctx, int kind, int64_t rows, int64_t cols, const void *X1, int64_t ld1, int64_t p1,
+              const void *X2, int64_t ld2, int64_t p2, void *Y, int64_t ldy);
 
 // elimination driver (elim.hip)
 enum ElimMode { ELIM_PLUQ = 0, ELIM_LU = 1, ELIM_RREF = 2 };

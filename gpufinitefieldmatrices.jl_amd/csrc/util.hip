@@ -1,5 +1,5 @@
 // util.hip -- elementwise kernels: mod, seeded rand, eye, storage conversion,
-// transposition-stack permutations.
+// transposition-stack permutations, two-modulus Chinese remaindering.
 #include "common.h"
 
 namespace {
@@ -170,6 +170,24 @@ int util_eye(modp_ctx *ctx, int kind, int64_t rows, int64_t cols, void *X, int64
     return MODP_OK;
 }
 
+// Y = the y in [0, p1 p2) with y = X1 mod p1 and y = X2 mod p2:
+// y = x1 + p1 * ((x2 - x1) * c mod p2), c = p1^{-1} mod p2 (Garner's step).
+template <typename T>
+__global__ void k_crt2(int64_t rows, int64_t cols, const T *X1, int64_t ld1, const T *X2, int64_t ld2, T *Y,
+                       int64_t ldy, ModP m1, ModP m2, double c)
+{
+    int64_t total = rows * cols;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = e % rows, j = e / rows;
+        const double x1 = (double)to_res<T>(X1[i + j * ld1], m1);
+        const double x2 = (double)to_res<T>(X2[i + j * ld2], m2);
+        const double d = dmod(x2 - dmod(x1, m2), m2);    // (x2 - x1) mod p2
+        const double t = dmod(d * c, m2);                 // < p2^2 <= 2^52: exact
+        Y[i + j * ldy] = (T)fma(m1.pd, t, x1);            // < p1 p2 <= 2^53: exact
+    }
+}
+
 template <typename TI>
 static int convert_from(modp_ctx *ctx, int64_t rows, int64_t cols, const TI *X, int64_t ldx, int kout,
                         void *Y, int64_t ldy, const ModP &mp)
@@ -225,4 +243,29 @@ int util_perm(modp_ctx *ctx, int kind, int rowwise, int64_t rows, int64_t cols, 
     // order by waiting for this one
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return MODP_OK;
+}
+
+int util_crt2(modp_ctx *ctx, int kind, int64_t rows, int64_t cols, const void *X1, int64_t ld1, int64_t p1,
+              const void *X2, int64_t ld2, int64_t p2, void *Y, int64_t ldy)
+{
+    // c = p1^{-1} mod p2 by the extended Euclidean algorithm on the host
+    int64_t r0 = p2, r1 = p1 % p2, s0 = 0, s1 = 1;
+    while (r1 != 0) {
+        const int64_t q = r0 / r1;
+        int64_t t = r0 - q * r1; r0 = r1; r1 = t;
+        t = s0 - q * s1; s0 = s1; s1 = t;
+    }
+    if (r0 != 1)
+        return set_error(ctx, MODP_ERR_ARG, "moduli %lld and %lld are not coprime", (long long)p1, (long long)p2);
+    const int64_t c = ((s0 % p2) + p2) % p2;
+    if (rows <= 0 || cols <= 0) return MODP_OK;
+    const unsigned g = grid_for(rows * cols);
+    const ModP m1 = make_modp(p1), m2 = make_modp(p2);
+    if (kind == EK_F64)
+        k_crt2<double><<<g, kThreads, 0, ctx->stream>>>(rows, cols, (const double *)X1, ld1, (const double *)X2, ld2,
+                                                         (double *)Y, ldy, m1, m2, (double)c);
+    else
+        k_crt2<float><<<g, kThreads, 0, ctx->stream>>>(rows, cols, (const float *)X1, ld1, (const float *)X2, ld2,
+                                                        (float *)Y, ldy, m1, m2, (double)c);
+    return hip_check(ctx, hipGetLastError(), "k_crt2 launch");
 }

@@ -598,6 +598,24 @@ def pluq(A: CuModArray, form: int = FORM_AUTO):
     return U, L, _stack(rs, nr.value), _stack(cs, nc.value), rk.value
 
 
+def crt(A1: CuModArray, A2: CuModArray) -> CuModArray:
+    """Chinese remaindering of two residue matrices of equal size: the
+    CuModArray of modulus N1*N2 whose entries reduce to A1 mod N1 and A2 mod N2
+    (the composite modulus of KaratsubaMatrix.jl:18-51, built from per-prime
+    results)."""
+    if A1.size != A2.size:
+        raise CuModArraySizeMismatchException("CRT needs matrices of the same size")
+    if A1.elem_type != A2.elem_type:
+        raise ArgumentError("Mismatched element types")
+    m, n = A1.size
+    N = A1.N * A2.N
+    Y = zeros(A1.elem_type, m, n, N, ctx=A1.ctx)
+    ctx = A1.ctx
+    ctx.check(ctx.lib.modp_crt2(ctx.h, A1.dtype_code, m, n, A1.ptr, A1.ld, A1.N, A2.ptr, A2.ld, A2.N, Y.ptr, Y.ld),
+              "crt")
+    return Y
+
+
 def _trtri(A: CuModArray, uplo: int) -> CuModArray:
     m, n = A.size
     Ainv = zeros(A.elem_type, n, m, A.N, ctx=A.ctx)

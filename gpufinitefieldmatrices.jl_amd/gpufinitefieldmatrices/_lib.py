@@ -97,6 +97,7 @@ SIGNATURES = [
     ("modp_lu", _int, [_vp, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64,
                        _p64, _p64, _p64]),
     ("modp_rref", _int, [_vp, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _p64, _p64]),
+    ("modp_crt2", _int, [_vp, _int, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64]),
     ("modp_trtri", _int, [_vp, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _i64]),
     ("modp_inverse", _int, [_vp, _int, _int, _i64, _vp, _i64, _vp, _i64, _i64, ctypes.POINTER(_int)]),
     ("modp_apply_row_perm", _int, [_vp, _int, _i64, _i64, _vp, _i64, _p64, _i64, _int]),

@@ -14,7 +14,9 @@ using LinearAlgebra
 
 export ModMatrix, CuModMatrix, pluq_gpu_kernel, pluq, lu_ple, rref, mod_inv,
        apply_row_perm!, apply_row_inv_perm!, apply_col_perm!, apply_col_inv_perm!,
-       stripe_mul!, find_max_stripe_ops, mat_mul_gpu_type, mat_mul_type_inplace!
+       stripe_mul!, find_max_stripe_ops, mat_mul_gpu_type, mat_mul_type_inplace!,
+       inverse, is_invertible, is_invertible_with_inverse,
+       upper_triangular_inverse_no_copy, lower_triangular_inverse_no_copy, crt
 
 const LIB = get(ENV, "MODP_HIP_LIB", joinpath(@__DIR__, "..", "gpufinitefieldmatrices", "libmodp_hip.so"))
 const TILE_WIDTH = 32
@@ -31,6 +33,12 @@ struct CuModMatrixModulusNotPrimeException <: Exception
     message::String
 end
 struct CuModMatrixNotSquareException <: Exception
+    message::String
+end
+struct InverseNotDefinedException <: Exception
+    message::String
+end
+struct MatrixNotInvertibleException <: Exception
     message::String
 end
 
@@ -264,5 +272,68 @@ apply_row_perm!(P, A::ModMatrix) = perm!(:modp_apply_row_perm, P, A, 0)
 apply_row_inv_perm!(P, A::ModMatrix) = perm!(:modp_apply_row_perm, P, A, 1)
 apply_col_perm!(P, A::ModMatrix) = perm!(:modp_apply_col_perm, P, A, 0)
 apply_col_inv_perm!(P, A::ModMatrix) = perm!(:modp_apply_col_perm, P, A, 1)
+
+# ---- inverses (triangular_inverse_no_copy.jl:197-229,450-478; CuModMatrix.jl:340-502) ----
+function triangular_inverse(A::ModMatrix{T}, uplo::Integer) where {T}
+    m, n = size(A)
+    Ainv = zeros(T, n, m, A.N)
+    st = ccall((:modp_trtri, LIB), Cint,
+               (Ptr{Cvoid}, Cint, Cint, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64),
+               context().h, dtype_code(T), uplo, m, n, A.data.ptr, ld(A), Ainv.data.ptr, ld(Ainv), A.N)
+    st == 1 && throw(InverseNotDefinedException("(Right) pseudoinverse not defined for tall lower triangular matrices"))
+    st == 4 && throw(MatrixNotInvertibleException("triangular matrix with a zero diagonal entry mod $(A.N)"))
+    check(st, "triangular inverse")
+    return Ainv
+end
+
+upper_triangular_inverse_no_copy(A::ModMatrix; debug::Bool=false) = triangular_inverse(A, 0)
+lower_triangular_inverse_no_copy(A::ModMatrix; debug::Bool=false) = triangular_inverse(A, 1)
+
+function inverse_or_nothing(A::ModMatrix{T}; form=FORM_AUTO) where {T}
+    m, n = size(A)
+    m == n || throw(CuModMatrixNotSquareException("Matrix must be square to be inverted"))
+    Ainv = zeros(T, n, n, A.N)
+    ok = Ref{Cint}(0)
+    check(ccall((:modp_inverse, LIB), Cint,
+                (Ptr{Cvoid}, Cint, Cint, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64, Ref{Cint}),
+                context().h, form, dtype_code(T), n, A.data.ptr, ld(A), Ainv.data.ptr, ld(Ainv), A.N, ok), "inverse")
+    return ok[] != 0 ? Ainv : nothing
+end
+
+function inverse(A::ModMatrix; form=FORM_AUTO)
+    X = inverse_or_nothing(A; form=form)
+    X === nothing && throw(MatrixNotInvertibleException("Matrix is not invertible mod $(A.N)"))
+    return X
+end
+
+function is_invertible_with_inverse(A::ModMatrix; debug::Bool=false)
+    X = inverse_or_nothing(A)
+    return X === nothing ? (false, nothing) : (true, X)
+end
+
+is_invertible(A::ModMatrix) = pluq(A)[5] == minimum(size(A))
+
+# ---- matrix-vector product (stripe_mul.jl:82-168, CuModMatrix.jl:816-836) ---------
+# a vector is held as an n x 1 ModMatrix
+function LinearAlgebra.mul!(z::ModMatrix{T}, A::ModMatrix{T}, x::ModMatrix{T}) where {T}
+    (A.N == z.N && A.N == x.N) || throw(CuModArrayModulusMismatchException("All matrices must have the same modulus N"))
+    (cols(A) == rows(x) && rows(z) == rows(A) && cols(x) == 1 && cols(z) == 1) ||
+        throw(CuModArraySizeMismatchException("Matrix dimensions do not match for multiplication"))
+    check(ccall((:modp_gemv, LIB), Cint,
+                (Ptr{Cvoid}, Cint, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64),
+                context().h, dtype_code(T), rows(A), cols(A), A.data.ptr, ld(A), x.data.ptr, z.data.ptr, A.N), "gemv")
+    return z
+end
+
+# ---- Chinese remaindering of per-prime results (composite modulus, KaratsubaMatrix.jl:18-51) ----
+function crt(A1::ModMatrix{T}, A2::ModMatrix{T}) where {T}
+    size(A1) == size(A2) || throw(CuModArraySizeMismatchException("CRT needs matrices of the same size"))
+    Y = zeros(T, rows(A1), cols(A1), A1.N * A2.N)
+    check(ccall((:modp_crt2, LIB), Cint,
+                (Ptr{Cvoid}, Cint, Int64, Int64, Ptr{Cvoid}, Int64, Int64, Ptr{Cvoid}, Int64, Int64, Ptr{Cvoid}, Int64),
+                context().h, dtype_code(T), rows(A1), cols(A1), A1.data.ptr, ld(A1), A1.N, A2.data.ptr, ld(A2), A2.N,
+                Y.data.ptr, ld(Y)), "crt")
+    return Y
+end
 
 end # module

@@ -199,6 +199,18 @@ int modp_apply_row_perm(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, vo
 int modp_apply_col_perm(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X, int64_t ldx,
                         const int64_t *stack, int64_t nswaps, int inverse);
 
+/* ---- multi-modular ------------------------------------------------------- */
+
+/* Chinese remaindering of two residue matrices: Y (rows x cols) receives the
+ * unique y in [0, p1*p2) with y = X1 mod p1 and y = X2 mod p2, for coprime
+ * p1, p2 <= 2^26 with p1*p2 <= 2^53 (Float64) or 2^24 (Float32) so that Y is
+ * exact in the storage type (a CuModArray of modulus p1*p2).  Combines the
+ * per-prime results of the batched forms; the reference's nearest
+ * counterpart is the two-limb composite modulus N1*N2 of KaratsubaMatrix
+ * (src/KaratsubaMatrix/KaratsubaMatrix.jl:18-51). */
+int modp_crt2(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, const void *X1, int64_t ld1, int64_t p1,
+              const void *X2, int64_t ld2, int64_t p2, void *Y, int64_t ldy);
+
 /* ---- inverses ------------------------------------------------------------ */
 
 /* Inverse of a triangular matrix mod p.  uplo 0: A (m x n) is upper

@@ -151,3 +151,33 @@ def test_inverse_4096_freivalds(gffm):
         x = rng.integers(0, p, size=n)
         y = O._mod_matvec(Xh, x, p)
         assert np.array_equal(O._mod_matvec(Ah, y, p), x)
+
+
+@pytest.mark.parametrize("p1,p2", [(2, 3), (65521, 65519), (67108859, 67108837), (8388593, 7)])
+def test_crt_two_moduli(gffm, p1, p2):
+    # per-prime products combined: CRT(A*B mod p1, A*B mod p2) = A*B mod p1*p2
+    # for small integer entries, against Python integers
+    g = gffm
+    rng = np.random.default_rng(p1 % 1000)
+    A = rng.integers(0, 50, size=(40, 30))
+    B = rng.integers(0, 50, size=(30, 20))
+    P1 = g.Array(dev(g, A, p1) * dev(g, B, p1)).astype(np.int64)
+    P2 = g.Array(dev(g, A, p2) * dev(g, B, p2)).astype(np.int64)
+    Y = g.crt(dev(g, P1, p1), dev(g, P2, p2))
+    assert Y.N == p1 * p2
+    exact = (A.astype(object) @ B.astype(object)) % (p1 * p2)
+    assert np.array_equal(g.Array(Y).astype(np.int64), np.asarray(exact, dtype=np.int64))
+    # random residue pairs: both congruences hold
+    X1 = rng.integers(0, p1, size=(17, 9))
+    X2 = rng.integers(0, p2, size=(17, 9))
+    Yh = g.Array(g.crt(dev(g, X1, p1), dev(g, X2, p2))).astype(np.int64)
+    assert (Yh % p1 == X1).all() and (Yh % p2 == X2).all() and (Yh < p1 * p2).all()
+
+
+def test_crt_errors(gffm):
+    g = gffm
+    X = dev(g, np.ones((3, 3), dtype=np.int64), 6)
+    with pytest.raises(g.ArgumentError):
+        g.crt(X, dev(g, np.ones((3, 3), dtype=np.int64), 4))     # not coprime
+    with pytest.raises(g.CuModArraySizeMismatchException):
+        g.crt(X, dev(g, np.ones((3, 4), dtype=np.int64), 5))
