@@ -1,6 +1,9 @@
 """Build libmodp_hip.so (HIP, gfx950) in-tree.
 
-    python gpufinitefieldmatrices.jl_amd/build.py [--jobs N]
+    python gpufinitefieldmatrices.jl_amd/build.py [--jobs N] [--only elim.hip ...]
+
+--only recompiles the named sources and relinks with the objects already in
+_build/ (a quick rebuild after editing one file that no other file includes).
 
 Each csrc/*.hip file is compiled separately (in parallel) with hipcc for
 gfx950 only, then linked into gpufinitefieldmatrices/libmodp_hip.so next to the
@@ -51,12 +54,19 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int | None = None) -> str:
+def build(jobs: int | None = None, only: list | None = None) -> str:
     os.makedirs(BUILD_DIR, exist_ok=True)
     srcs = _sources()
-    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
+    todo = srcs if not only else [s for s in srcs if s in only]
+    if only and len(todo) != len(only):
+        raise RuntimeError(f"unknown sources in {only}")
+    jobs = jobs or min(len(todo), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    objs = [os.path.join(BUILD_DIR, s.replace(".hip", ".o")) for s in srcs]
+    missing = [o for o in objs if not os.path.exists(o)]
+    if missing:
+        raise RuntimeError(f"objects missing (run a full build): {missing}")
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -69,5 +79,6 @@ def build(jobs: int | None = None) -> str:
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--only", nargs="+", default=None)
     a = ap.parse_args()
-    print(build(a.jobs))
+    print(build(a.jobs, a.only))

@@ -12,7 +12,7 @@
 // iteration with a host round-trip each; here one launch factors a whole
 // panel):
 //   for each inner panel of w columns (w*rows-per-thread <= 64 registers):
-//     k_panel    one 1024-thread workgroup holds the panel in registers; per
+//     k_panel    one 512-thread workgroup holds the panel in registers; per
 //                column: wave64 argmax by __shfl_xor with lowest-row
 //                tie-break, cross-wave through LDS, modular inverse of the
 //                pivot by extended Euclid, row swap, normalisation and the
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(256) void k_finish(int32_t *__restrict__ W, int64_t
     }
 }
 
-constexpr int MAXPAN = 32;  // inner panels per outer block (NBO / smallest panel width)
+constexpr int MAXPAN = 64;  // inner panels per outer block (NBO / smallest panel width, 2)
 
 // End of an outer block with P pivots (rows [KR, KR+P)), whose inner panels
 // left their row gather maps in bmap_*: one wave per column outside the block
@@ -736,18 +736,26 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
     if (m > 16384)
         return set_error(ctx, MODP_ERR_DIM, "elimination supports at most 16384 rows (got %lld)", (long long)m);
 
-    // panel shape: rows per thread x panel width <= 64 registers
-    int RPT = 1;
-    while ((int64_t)RPT * PT < m) RPT *= 2;
-    // panel width per rows-per-thread, measured at 2048 and 4096 rows
-    int NBW = RPT == 1 ? 32 : (RPT == 2 ? 16 : (RPT <= 8 ? 8 : 2));
-    if (const char *e = getenv("MODP_PANEL_W")) {
+    // Panel shape of an outer block starting at row KR: the fewest rows per
+    // thread that hold every row from KR's 4-aligned row down (the panel
+    // kernel reads 4-row vectors from kr0 & ~3 >= KR & ~3), and the panel
+    // width for that many rows per thread (rows per thread x width <= 64
+    // registers; widths measured at 2048 and 4096 rows).  Late blocks have
+    // fewer rows, so their panels are shorter and wider.
+    const char *wenv = getenv("MODP_PANEL_W");
+    const int want_w = wenv ? atoi(wenv) : 0;
+    auto panel_shape = [&](int64_t KR, int *RPT, int *NBW) {
+        const int64_t need = m - (KR & ~(int64_t)3);
+        int r = 1;
+        while ((int64_t)r * PT < need) r *= 2;
+        int wd = r == 1 ? 32 : (r == 2 ? 16 : (r <= 8 ? 8 : 2));
         // tuning override among the instantiated panel widths
-        const int want = atoi(e);
-        if ((RPT <= 2 && (want == 16 || want == 32)) || (RPT == 4 && (want == 8 || want == 16)) ||
-            (RPT == 8 && (want == 4 || want == 8)))
-            NBW = want;
-    }
+        if ((r <= 2 && (want_w == 16 || want_w == 32)) || (r == 4 && (want_w == 8 || want_w == 16)) ||
+            (r == 8 && (want_w == 4 || want_w == 8)))
+            wd = want_w;
+        *RPT = r;
+        *NBW = wd;
+    };
 
     int st = MODP_OK;
     const int64_t ldw = rup(m, 64);
@@ -790,7 +798,8 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
     const int big_form = form == MODP_FORM_AUTO ? choose_form(m, n, NBO, p) : form;
 
     // panel t of the current outer block at (rj, cj), width w
-    auto launch_panel_t = [&](int t, int64_t rj, int64_t cj, int64_t w, int *abort) -> int {
+    auto launch_panel_t = [&](int RPT, int NBW, int t, int64_t rj, int64_t cj, int64_t w, int *abort) -> int {
+        if (t >= MAXPAN) return set_error(ctx, MODP_ERR_DEVICE, "more than %d panels in an outer block", MAXPAN);
         int32_t *mdst = bmap_dst + t * 64, *msrc = bmap_src + t * 64;
         PanelCtl *pctl = bctl + t;
         const int key = RPT * 100 + NBW;
@@ -859,6 +868,8 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
         if (nbo > NBO) nbo = NBO;
         if (nbo < 0) nbo = 0;  // pluq: only moved zero columns remain
         const int64_t FAR = KC + nbo;
+        int RPT, NBW;
+        panel_shape(KR, &RPT, &NBW);
         bool stop_zero = false;
         if (!ple && nbo > 0) {
             // pluq: every panel of the block is enqueued assuming it finds a
@@ -879,7 +890,7 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
             const int np = (int)pr.size();
             int64_t Pfull = 0;
             for (int t = 0; t < np; ++t) {
-                MODP_TRY(launch_panel_t(t, pr[t], pcj[t], pw[t], abort_flag));
+                MODP_TRY(launch_panel_t(RPT, NBW, t, pr[t], pcj[t], pw[t], abort_flag));
                 MODP_TRY(finish_t(t, KC, FAR, pr[t], pcj[t], pw[t], pw[t], abort_flag));
                 Pfull += pw[t];
             }
@@ -918,7 +929,7 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
                 const int64_t cj = kc, rj = kr;
                 int64_t w = FAR - cj;
                 if (w > NBW) w = NBW;
-                MODP_TRY(launch_panel_t(npanels, rj, cj, w, nullptr));
+                MODP_TRY(launch_panel_t(RPT, NBW, npanels, rj, cj, w, nullptr));
                 HIP_TRY(ctx, hipMemcpyAsync(hctl, bctl + npanels, sizeof(PanelCtl), hipMemcpyDeviceToHost, ctx->stream));
                 HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
                 const PanelCtl pc = *hctl;

@@ -90,6 +90,23 @@ def test_pluq_medium_vs_oracle(gffm, n, r, form):
     check_pluq_equal(gffm, Ah, p, form=form)
 
 
+@pytest.mark.parametrize("shape,r", [((4200, 200), 200), ((4300, 150), 90), ((9000, 140), 140)], ids=str)
+def test_pluq_lu_tall_vs_oracle(gffm, shape, r):
+    # more than 4096 rows: two-column panels, 64 panels per outer block
+    g = gffm
+    p = 65521
+    m, n = shape
+    print(f"tall pluq/lu shape={shape} r={r}", flush=True)
+    Ah = O.draw_matrix(m, n, p, seed=9) if r == n else rank_deficient(m, n, r, p, seed=9)
+    check_pluq_equal(g, Ah, p, form=1)
+    U, L, perm, piv, rank = g.lu_full(dev(g, Ah, p))
+    ref = O.lu(Ah, p)
+    assert rank == ref["rank"] == r
+    assert perm == ref["perm"].tolist() and piv == ref["pivcols"].tolist()
+    assert np.array_equal(g.Array(U).astype(np.int64), ref["U"])
+    assert np.array_equal(g.Array(L).astype(np.int64), ref["L"])
+
+
 def test_pluq_float32_storage(gffm):
     for p in (7, 251, 65521):
         Ah = O.draw_matrix(60, 45, p, seed=4)
