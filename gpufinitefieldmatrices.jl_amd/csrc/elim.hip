@@ -121,6 +121,17 @@ template <bool SMALL> __device__ __forceinline__ uint32_t mulmod_t(uint32_t a, u
     return (uint32_t)dmod((double)a * (double)b, mp);
 }
 
+// x mod p for exact integers |x| <= 2^51: fl(x * pinv) is within
+// |x| 2^-52 / p <= 1/(2p) of x/p, so q = rint(.) is within 1/2 + 1/(2p) < 1 of
+// x/p, r = x - q p is exact with |r| < p, and one fix-up brings it to [0,p).
+__device__ __forceinline__ double dmod_near(double x, const ModP &mp)
+{
+    const double r = fma(-rint(x * mp.pinv), mp.pd, x);
+    return r < 0.0 ? r + mp.pd : r;
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // Pivot-search keys: larger key = larger residue, then lower position.
 template <bool SMALL> struct PivKey;
 template <> struct PivKey<true> {
@@ -210,7 +221,7 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
     typedef PivKey<SMALL> K;
     typedef typename K::T key_t;
     __shared__ key_t s_key[2][NW];
-    __shared__ uint32_t s_crow_val[2][NW][NBW];  // candidate rows (residues)
+    __shared__ double s_crow[2][NW][NBW];        // candidate rows (unreduced)
     __shared__ uint32_t s_cinv[2][NW];
     __shared__ int s_cidx[2][NW];                // candidate local row index
     __shared__ uint32_t s_U[NBW][NBW];
@@ -249,27 +260,39 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
     for (int q = 0; q < RPT; ++q) pos[q] = rowi(q);
     if (tid == 0) s_cnt = 0;
 
+    // rows that exist (bit q: register row q is a panel row)
+    uint32_t live = 0;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int i = rowi(q);
+        if (i >= 0 && i < rows) live |= 1u << q;
+    }
     int pc = 0, status = 0, consumed = w, par = 0;
     bool active = true;
-    // one step per panel column, unrolled at compile time so that v[][jc]
-    // is a register
+    // One step per panel column, unrolled at compile time so that v[][jc]
+    // is a register.  Every step runs, also after the panel has stopped or
+    // past its width, with a zero update: the step is straight-line code,
+    // so the register array is never copied between branches.  Per-thread
+    // work is kept to the rows the thread owns: the pivot row is published
+    // unreduced, and its reduction and normalisation are spread over the
+    // lanes (lane j handles column j) and broadcast back with readlane.
     auto step = [&](auto JC) __attribute__((always_inline)) {
         constexpr int jc = decltype(JC)::value;
-        if (!active || jc >= w) return;
-        if (pc >= rows) {
+        const bool run = active && jc < w && pc < rows;
+        if (active && jc < w && pc >= rows) {
             status = 2;
             consumed = jc;
             active = false;
-            return;
         }
-        // this thread's best row among those not yet pivots
+        // this thread's best row among those not yet pivots; column jc is
+        // final after this step, so it is reduced here once and for all
+        const uint32_t elig = run ? live & ~pivmask : 0u;
         key_t mine = 0;
         int mq = 0;
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            if (SMALL) v[q][jc] = dmod(v[q][jc], mp);
-            const int i = rowi(q);
-            if (i >= 0 && i < rows && !((pivmask >> q) & 1u)) {
+            if (SMALL) v[q][jc] = dmod_near(v[q][jc], mp);
+            if ((elig >> q) & 1u) {
                 const key_t k = K::make((uint32_t)v[q][jc], pos[q]);
                 if (k > mine) { mine = k; mq = q; }
             }
@@ -295,7 +318,7 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
                         x = q == mq ? v[q][j] : x;
                         asm volatile("" : "+v"(x));
                     }
-                    s_crow_val[par][wave][j] = (uint32_t)dmod(x, mp);
+                    s_crow[par][wave][j] = x;
                 }
         }
         if (lane == 0) s_key[par][wave] = wbest;
@@ -307,11 +330,15 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
             const key_t k = s_key[par][x];
             if (k > best) { best = k; bw = x; }
         }
+        bw = uniform(bw);
         const int cp = par;
         par ^= 1;
-        const uint32_t V = K::val(best);
-        const int Ip = K::pos(best);
-        if (V == 0) {
+        const uint32_t V = (uint32_t)uniform((int)K::val(best));
+        const int Ip = uniform(K::pos(best));
+        const uint32_t inv = V != 0 ? (uint32_t)uniform((int)s_cinv[cp][bw]) : 0u;
+        const int wrow = V != 0 ? uniform(s_cidx[cp][bw]) : -1;
+        const bool piv = run && V != 0 && inv != 0;
+        if (run && V == 0) {
             if (ple) {
                 if (tid == 0) colmap[jc] = -1;
             } else {
@@ -319,36 +346,39 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
                 consumed = jc;
                 active = false;
             }
-            return;
         }
-        const uint32_t inv = s_cinv[cp][bw];
-        if (inv == 0) {
+        if (run && V != 0 && inv == 0) {
             status = 3;
             consumed = jc;
             active = false;
-            return;
         }
-        const int wrow = s_cidx[cp][bw];
         // positions: the winner moves to pc, the row at pc moves to Ip
+        if (piv) {
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            const int i = rowi(q);
-            if (i == wrow) { pos[q] = pc; pivmask |= 1u << q; }
-            else if (pos[q] == pc) pos[q] = Ip;
+            for (int q = 0; q < RPT; ++q) {
+                const bool win = rowi(q) == wrow;
+                pivmask |= win ? 1u << q : 0u;
+                pos[q] = win ? pc : (pos[q] == pc ? Ip : pos[q]);
+            }
         }
-        // normalised pivot row, then the rank-1 update of the rows that are
-        // not pivots (frozen and out-of-range rows get multiplier 0)
+        // normalised pivot row: lane j reduces and scales column j, wave 0
+        // keeps it for the output, readlane hands every lane the values
+        uint32_t ul = 0;
+        if (piv && lane > jc && lane < NBW) {
+            const double x = s_crow[cp][bw][lane];
+            const uint32_t r = (uint32_t)(SMALL ? dmod_near(x, mp) : x);
+            ul = mulmod_t<SMALL>(r, inv, mp);
+            if (wave == 0) s_U[pc][lane] = ul;
+        }
         double u[NBW];
 #pragma unroll
-        for (int j = 0; j < NBW; ++j) {
-            const uint32_t uj = j > jc ? mulmod_t<SMALL>(s_crow_val[cp][bw][j], inv, mp) : 0u;
-            if (tid == 0 && j > jc) s_U[pc][j] = uj;
-            u[j] = (double)uj;
-        }
+        for (int j = 0; j < NBW; ++j) u[j] = j > jc ? (double)(uint32_t)__builtin_amdgcn_readlane((int)ul, j) : 0.0;
+        // rank-1 update of the rows that are not pivots (a zero update when
+        // this step found no pivot)
+        const uint32_t upd = live & ~pivmask;
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            const int i = rowi(q);
-            const double l = (i >= 0 && i < rows && !((pivmask >> q) & 1u)) ? v[q][jc] : 0.0;
+            const double l = ((upd >> q) & 1u) ? v[q][jc] : 0.0;
 #pragma unroll
             for (int j = 0; j < NBW; ++j)
                 if (j > jc) {
@@ -356,7 +386,7 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
                     if (!SMALL) v[q][j] = dmod(v[q][j], mp);
                 }
         }
-        if (tid == 0) {
+        if (piv && tid == 0) {
             ipiv[kr0 + pc] = kr0 + Ip;
             pivval[kr0 + pc] = (int32_t)V;
             pivinv[kr0 + pc] = (int32_t)inv;
@@ -364,7 +394,7 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
             colmap[jc] = pc;
             s_pcol[pc] = jc;
         }
-        ++pc;
+        pc += piv ? 1 : 0;
     };
     Unroll<0, NBW>::run(step);
     __syncthreads();
@@ -389,7 +419,7 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
                 if (j < w) {
                     int32_t x[VW];
 #pragma unroll
-                    for (int e = 0; e < VW; ++e) x[e] = (int32_t)(uint32_t)dmod(v[g * VW + e][j], mp);
+                    for (int e = 0; e < VW; ++e) x[e] = (int32_t)(uint32_t)v[g * VW + e][j];
                     store_rows<VW>(W + base + (int64_t)(c0 + j) * ldw, x);
                 }
             continue;
@@ -405,11 +435,11 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
                 for (int j = 0; j < NBW; ++j)
                     if (j < w)
                         W[dst + (int64_t)(c0 + j) * ldw] =
-                            (int32_t)(j < jk ? (uint32_t)dmod(v[q][j], mp) : (j == jk ? 1u : s_U[k][j]));
+                            (int32_t)(j < jk ? (uint32_t)v[q][j] : (j == jk ? 1u : s_U[k][j]));
             } else {
 #pragma unroll
                 for (int j = 0; j < NBW; ++j)
-                    if (j < w) W[dst + (int64_t)(c0 + j) * ldw] = (int32_t)(uint32_t)dmod(v[q][j], mp);
+                    if (j < w) W[dst + (int64_t)(c0 + j) * ldw] = (int32_t)(uint32_t)v[q][j];
             }
         }
     }
