@@ -13,19 +13,18 @@
 // panel):
 //   for each inner panel of w columns (w*rows-per-thread <= 64 registers):
 //     k_panel    one 512-thread workgroup holds the panel in registers; per
-//                column: wave64 argmax by __shfl_xor with lowest-row
-//                tie-break, cross-wave through LDS, modular inverse of the
-//                pivot by extended Euclid, row swap, normalisation and the
-//                rank-1 update of the remaining panel columns.
-//     k_finish   one wave per column outside the panel: the panel's row
-//                swaps (as a precomputed gather map), then for columns right
-//                of the panel the triangular solve with the panel's L11
-//                (lanes hold the column, shuffles broadcast the solved rows),
-//                writing U's rows into the outer-block buffer Ubig.
-//     G1, G2     GEMM updates (EPI_SUB) of the rest of the outer block (all
-//                rows) and of the rows of the far columns that are still in
-//                this outer block.
-//   after the outer block: one GEMM with K = NBO for the far trailing block.
+//                column: wave64 argmax by DPP with lowest-row tie-break,
+//                cross-wave through LDS, the pivot's inverse from a table (or
+//                extended Euclid), and the rank-1 update of the remaining
+//                panel columns; rows move only at the end, by a row map, and
+//                the old values of the moved rows in the block's other
+//                columns are staged.
+//     k_update   the block's other columns: the row map, and right of the
+//                panel the triangular solve with L11 (rows of U) and the
+//                rank-w update of every row below the pivots.
+//   after the outer block: k_block_finish applies the block's row maps and
+//   the solve with the block's L11 to the far columns, then one GEMM with
+//   K = NBO (limb or plain form) updates the far trailing block.
 #include <stdlib.h>
 
 #include <type_traits>
@@ -205,7 +204,7 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
                                               int32_t *__restrict__ pivinv, int32_t *__restrict__ pivcol,
                                               int32_t *__restrict__ colmap, int32_t *__restrict__ map_dst,
                                               int32_t *__restrict__ map_src, PanelCtl *__restrict__ ctl,
-                                              int *__restrict__ abort)
+                                              int *__restrict__ abort, int KC, int FAR, int32_t *__restrict__ stage)
 {
     constexpr int NW = PT / 64;
     if (abort && *abort) {
@@ -227,6 +226,7 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
     __shared__ uint32_t s_U[NBW][NBW];
     __shared__ int s_pcol[NBW];
     __shared__ int s_cnt;
+    __shared__ int s_msrc[2 * NBW];  // source row of each row-map entry
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rows = m - kr0;
@@ -301,25 +301,30 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
         // reduction, so the table load overlaps it
         const uint32_t myinv = (invtab && K::val(mine) != 0) ? invtab[K::val(mine)] : 0u;
         const key_t wbest = wave_max(mine);
-        if (mine == wbest && K::val(wbest) != 0) {
-            // this lane holds the wave's candidate: publish row, index, inverse
-            const uint32_t V = K::val(wbest);
-            s_cinv[par][wave] = invtab ? myinv : modinv_fast(V, mp);
-            s_cidx[par][wave] = rowi(mq);
+        // the lane holding the wave's candidate (keys are unique) publishes
+        // its index and inverse; its register row is made wave-uniform, so
+        // the row is published through a scalar branch per register row
+        // rather than a per-lane select over all of them
+        const uint64_t wmask = __ballot(mine == wbest && K::val(wbest) != 0);
+        if (wmask) {
+            const int wl = (int)__builtin_ctzll(wmask);
+            const int wq = __builtin_amdgcn_readlane(mq, wl);
+            if (lane == wl) {
+                s_cinv[par][wave] = invtab ? myinv : modinv_fast(K::val(wbest), mp);
+                s_cidx[par][wave] = rowi(mq);
+            }
+            auto publish = [&](auto Q) __attribute__((always_inline)) {
+                constexpr int q = decltype(Q)::value;
+                if (wq == q && lane == wl) {
 #pragma unroll
-            for (int j = 0; j < NBW; ++j)
-                if (j > jc) {
-                    // register select of row mq (an empty asm keeps the
-                    // compiler from turning the chain into an indexed load,
-                    // which would move v[][] to scratch memory)
-                    double x = v[0][j];
-#pragma unroll
-                    for (int q = 1; q < RPT; ++q) {
-                        x = q == mq ? v[q][j] : x;
-                        asm volatile("" : "+v"(x));
-                    }
-                    s_crow[par][wave][j] = x;
+                    for (int j = 0; j < NBW; ++j)
+                        if (j > jc) s_crow[par][wave][j] = v[q][j];
+                    // keeps the cases' stores apart: merged, they would
+                    // become one store of an indexed (scratch) register row
+                    asm volatile("" ::: "memory");
                 }
+            };
+            Unroll<0, RPT>::run(publish);
         }
         if (lane == 0) s_key[par][wave] = wbest;
         __syncthreads();
@@ -399,11 +404,47 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
     Unroll<0, NBW>::run(step);
     __syncthreads();
 
+    // Row gather map: positions 0..pc-1 first, in order, then every other
+    // row that moved.
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int i = rowi(q);
+        if (i >= 0 && i < rows) {
+            const int dst = kr0 + pos[q];
+            if (pos[q] < pc) {
+                map_dst[pos[q]] = dst;
+                map_src[pos[q]] = kr0 + i;
+                s_msrc[pos[q]] = kr0 + i;
+            } else if (pos[q] != i) {
+                const int e = pc + atomicAdd(&s_cnt, 1);
+                map_dst[e] = dst;
+                map_src[e] = kr0 + i;
+                s_msrc[e] = kr0 + i;
+            }
+        }
+    }
+    __syncthreads();
+    // The old values of every moved row in the block's other columns go to
+    // the stage (stage[e + 64 ci], ci counting the block columns left of the
+    // panel, then those right of it) for the update kernel; the loads are
+    // issued here and stored after the panel's own stores.
+    constexpr int SPT = (2 * NBW * NBO + PT - 1) / PT;
+    const int nmap_all = pc + s_cnt, ncol_other = (FAR - KC) - w, nstage = nmap_all * ncol_other;
+    int32_t sv[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+        const int idx = tid + k * PT;
+        if (idx < nstage) {
+            const int e = idx % nmap_all, ci = idx / nmap_all;
+            const int c = ci < c0 - KC ? KC + ci : KC + ci + w;
+            sv[k] = W[(int64_t)s_msrc[e] + (int64_t)c * ldw];
+        }
+    }
+
     // every row to its final position; pivot rows take 1 at their pivot
     // column and the normalised U row after it.  A vector of VW rows none of
     // which moved is stored with one vector store per column; other rows one
-    // by one.  Row gather map: positions 0..pc-1 first, in order, then every
-    // other row that moved.
+    // by one.
 #pragma unroll
     for (int g = 0; g < RPT / VW; ++g) {
         bool still = true;
@@ -444,21 +485,10 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
         }
     }
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        const int i = rowi(q);
-        if (i >= 0 && i < rows) {
-            const int dst = kr0 + pos[q];
-            if (pos[q] < pc) {
-                map_dst[pos[q]] = dst;
-                map_src[pos[q]] = kr0 + i;
-            } else if (pos[q] != i) {
-                const int e = pc + atomicAdd(&s_cnt, 1);
-                map_dst[e] = dst;
-                map_src[e] = kr0 + i;
-            }
-        }
+    for (int k = 0; k < SPT; ++k) {
+        const int idx = tid + k * PT;
+        if (idx < nstage) stage[idx % nmap_all + 64 * (idx / nmap_all)] = sv[k];
     }
-    __syncthreads();
     if (tid == 0) {
         for (int j = consumed; j < w; ++j) colmap[j] = -1;
         ctl->npiv = pc;
@@ -469,61 +499,166 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
     }
 }
 
-// One wave per column of the outer block [ca, cb) outside the panel
-// [c0, c0+w): apply the panel's row swaps (a gather map whose first npiv
-// entries are the pivot rows in order); for columns right of the panel also
-// solve with the panel's L11 and write the U rows into Ubig (rows indexed by
-// pivot column - KC, columns by c - KC).
-__global__ __launch_bounds__(256) void k_finish(int32_t *__restrict__ W, int64_t ldw, int64_t ca, int64_t cb,
-                                                int kr0, int c0, int w, int KC, const PanelCtl *__restrict__ ctl,
+// After panel (kr0, c0, w) of the outer block [KC, FAR), in one launch:
+// the panel's row moves for every other column of the block, and for the
+// columns right of the panel the triangular solve with L11 (their rows of U)
+// and the rank-npiv update of every row below the pivots.  The old values
+// of moved rows come from the stage the panel kernel filled before anything
+// moved, so no block reads a row that another block writes.  Grid: x =
+// 64-row tiles from kr0 down, y = 32-column groups, the groups left of the
+// panel first (moves only, done by the blocks of the first row tile).
+// LAZY (p < 2^16): at most 32 products below 2^32 are summed exactly in FP64
+// before one reduction; otherwise every product is reduced.
+constexpr int UT_R = 64, UT_C = 32;
+template <int WMAX, bool LAZY>
+__global__ __launch_bounds__(256) void k_update(int32_t *__restrict__ W, int64_t ldw, int m, int KC, int FAR,
+                                                int kr0, int c0, int w, const PanelCtl *__restrict__ ctl,
                                                 const int32_t *__restrict__ pivinv,
                                                 const int32_t *__restrict__ pivcol,
-                                                const int32_t *__restrict__ colmap,
                                                 const int32_t *__restrict__ map_dst,
-                                                const int32_t *__restrict__ map_src, int32_t *__restrict__ Ubig,
-                                                int64_t ldub, ModP mp, const int *__restrict__ skip)
+                                                const int32_t *__restrict__ stage, ModP mp,
+                                                const int *__restrict__ skip)
 {
-    if (skip && *skip) return;
-    __shared__ uint32_t Ls[64][65];
-    __shared__ uint32_t invs[64];
-    __shared__ int pcols[64];
+    // the skip flag and the panel's counts are read together
+    const int skipped = skip ? *skip : 0;
     const int npiv = ctl->npiv, nmap = ctl->nmap;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int idx = tid; idx < npiv * npiv; idx += 256) {
-        const int i = idx / npiv, j = idx % npiv;
-        Ls[i][j] = j < i ? (uint32_t)W[(int64_t)(kr0 + i) + (int64_t)pivcol[kr0 + j] * ldw] : 0u;
+    if (skipped) return;
+    constexpr int CPT = UT_C / 4;            // columns per thread
+    __shared__ uint32_t Lf[WMAX][WMAX + 1];  // the panel's top w x w block
+    __shared__ uint32_t Ls[WMAX][WMAX + 1];  // L11 (row i, pivot j < i)
+    __shared__ uint32_t invs[WMAX];
+    __shared__ int s_pj[WMAX];              // panel column of pivot i
+    __shared__ int s_dst[64];               // row-map destinations
+    __shared__ uint32_t Ut[WMAX][UT_C];      // U rows by panel column (0: no pivot)
+    __shared__ int s_e[UT_R];               // tile row -> row-map entry, or -1
+    const int tid = threadIdx.x, cl = tid >> 6;
+    const int nl = c0 - KC, nr = FAR - (c0 + w);
+    const int gl = (nl + UT_C - 1) / UT_C;
+    const int g = blockIdx.y;
+    if (g < gl) {
+        // left of the panel: the row moves only
+        if (blockIdx.x != 0) return;
+        const int cb = g * UT_C, nc = min(UT_C, nl - cb);
+        for (int idx = tid; idx < nmap * nc; idx += 256) {
+            const int e = idx % nmap, ci = cb + idx / nmap;
+            W[(int64_t)map_dst[e] + (int64_t)(KC + ci) * ldw] = stage[e + 64 * ci];
+        }
+        return;
     }
-    if (tid < npiv) {
-        invs[tid] = (uint32_t)pivinv[kr0 + tid];
-        pcols[tid] = pivcol[kr0 + tid];
+    const int cb = (g - gl) * UT_C, nc = min(UT_C, nr - cb);
+    const int ci0 = nl + cb;        // stage column of the group's first column
+    const int cfirst = c0 + w + cb;  // its column in W
+    const int64_t r0 = (int64_t)kr0 + (int64_t)blockIdx.x * UT_R;
+    const int64_t r = r0 + (tid & 63);
+    const bool row_ok = r < m;
+
+    // every global load is issued before anything waits for one: the
+    // panel's top block, pivot columns and inverses, the row map, the
+    // solve's inputs (thread k < nc: column k of the pivot rows' old
+    // values), this thread's row of L and its old values.  Indices are
+    // clamped into range so that the loads are unconditional; values that
+    // are not needed are masked when they are used.
+    constexpr int FPT = (WMAX * WMAX + 255) / 256;
+    const int wl = w - 1;
+    const int64_t rr = row_ok ? r : (int64_t)kr0;
+    int32_t lf[FPT];
+#pragma unroll
+    for (int k = 0; k < FPT; ++k) {
+        const int idx = tid + 256 * k, i = min(idx / WMAX, wl), jj = min(idx % WMAX, wl);
+        lf[k] = W[(int64_t)(kr0 + i) + (int64_t)(c0 + jj) * ldw];
+    }
+    const int tw = min(tid, WMAX - 1);
+    const int32_t iv = pivinv[kr0 + tw];
+    const int32_t pjv = pivcol[kr0 + tw];
+    const int32_t mdv = map_dst[min(tid, 63)];
+    int32_t xs0[WMAX];
+    const int64_t xcol = 64 * (int64_t)(ci0 + min(tid, nc - 1));
+#pragma unroll
+    for (int i = 0; i < WMAX; ++i) xs0[i] = stage[min(i, wl) + xcol];
+    int32_t lr0[WMAX];
+#pragma unroll
+    for (int jj = 0; jj < WMAX; ++jj) lr0[jj] = W[rr + (int64_t)(c0 + min(jj, wl)) * ldw];
+    int32_t cv[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) cv[k] = W[rr + (int64_t)(cfirst + min(cl + 4 * k, nc - 1)) * ldw];
+    for (int idx = tid; idx < WMAX * UT_C; idx += 256) Ut[idx / UT_C][idx % UT_C] = 0u;
+    if (tid < UT_R) s_e[tid] = -1;
+#pragma unroll
+    for (int k = 0; k < FPT; ++k) {
+        const int idx = tid + 256 * k;
+        if (idx < WMAX * WMAX) Lf[idx / WMAX][idx % WMAX] = (idx / WMAX < w && idx % WMAX < w) ? (uint32_t)lf[k] : 0u;
+    }
+    if (tid < WMAX) {
+        invs[tid] = (uint32_t)iv;
+        s_pj[tid] = pjv - c0;
+    }
+    if (tid < 64) s_dst[tid] = mdv;
+    uint32_t xs[WMAX];
+#pragma unroll
+    for (int i = 0; i < WMAX; ++i) xs[i] = i < w ? (uint32_t)xs0[i] : 0u;
+    double Lr[WMAX];
+#pragma unroll
+    for (int jj = 0; jj < WMAX; ++jj) Lr[jj] = jj < w ? (double)lr0[jj] : 0.0;
+    __syncthreads();
+    if (tid >= npiv && tid < nmap) {
+        const int64_t d = (int64_t)s_dst[tid] - r0;
+        if (d >= 0 && d < UT_R) s_e[d] = tid;
+    }
+    for (int idx = tid; idx < WMAX * WMAX; idx += 256) {
+        const int i = idx / WMAX, j = idx % WMAX;
+        Ls[i][j] = (j < i && i < npiv) ? Lf[i][s_pj[j]] : 0u;
     }
     __syncthreads();
-    const int64_t ncols = (cb - ca) - w;
-    const int d0 = lane < nmap ? map_dst[lane] : 0, s0 = lane < nmap ? map_src[lane] : 0;
-    const bool zero_row = lane < w && colmap[lane] < 0;
-    for (int64_t ci = (int64_t)blockIdx.x * 4 + wave; ci < ncols; ci += (int64_t)gridDim.x * 4) {
-        int64_t c = ca + ci;
-        if (c >= c0) c += w;
-        int32_t *col = W + c * ldw;
-        uint32_t v0 = 0;
-        if (lane < nmap) v0 = (uint32_t)col[s0];
-        if (c < c0) {
-            if (lane < nmap) col[d0] = (int32_t)v0;
-            continue;
+    // rows of U for the group's columns: thread k solves column k
+    if (tid < nc) {
+#pragma unroll
+        for (int i = 0; i < WMAX; ++i) {
+            if (i < npiv) {
+                double acc = (double)xs[i];
+#pragma unroll
+                for (int j = 0; j < i; ++j) {
+                    acc = fma(-(double)Ls[i][j], (double)xs[j], acc);
+                    if (!LAZY) acc = dmod(acc, mp);
+                }
+                xs[i] = mulmod((uint32_t)dmod(acc, mp), invs[i], mp);
+                Ut[s_pj[i]][tid] = xs[i];
+            }
         }
-        if (lane >= npiv && lane < nmap) col[d0] = (int32_t)v0;
-        uint32_t x = v0;
-        for (int j = 0; j < npiv; ++j) {
-            if (lane == j) x = mulmod(x, invs[j], mp);
-            const uint32_t xj = (uint32_t)__shfl((int)x, j);
-            if (lane > j && lane < npiv) x = submod(x, mulmod(Ls[lane][j], xj, mp), mp);
+    }
+    __syncthreads();
+    if (!row_ok) return;
+    const int rel = (int)(r - kr0);
+    if (rel < npiv) {
+        // pivot row: its row of U
+        const int pj = s_pj[rel];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int col = cl + 4 * k;
+            if (col < nc) W[r + (int64_t)(cfirst + col) * ldw] = (int32_t)Ut[pj][col];
         }
-        int32_t *ucol = Ubig + (c - KC) * ldub;
-        if (lane < npiv) {
-            col[kr0 + lane] = (int32_t)x;
-            ucol[pcols[lane] - KC] = (int32_t)x;
+        return;
+    }
+    const int e = s_e[tid & 63];
+    if (e >= 0) {
+        // a row that moved here: its old values are in the stage
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int col = cl + 4 * k;
+            if (col < nc) cv[k] = stage[e + 64 * (ci0 + col)];
         }
-        if (zero_row) ucol[c0 - KC + lane] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int col = cl + 4 * k;
+        if (col < nc) {
+            double acc = (double)cv[k];
+#pragma unroll
+            for (int jj = 0; jj < WMAX; ++jj) {
+                acc = fma(-Lr[jj], (double)Ut[jj][col], acc);
+                if (!LAZY) acc = dmod(acc, mp);
+            }
+            W[r + (int64_t)(cfirst + col) * ldw] = (int32_t)(uint32_t)(LAZY ? dmod_near(acc, mp) : acc);
+        }
     }
 }
 
@@ -694,16 +829,19 @@ __global__ __launch_bounds__(256) void k_trsm_upper_unit(int32_t *__restrict__ W
 template <int RPT, int NBW>
 int launch_panel(modp_ctx *ctx, int32_t *W, int64_t ldw, int m, int kr0, int c0, int w, const ModP &mp, int ple,
                  const uint32_t *invtab, int32_t *ipiv, int32_t *pivval, int32_t *pivinv, int32_t *pivcol,
-                 int32_t *colmap, int32_t *map_dst, int32_t *map_src, PanelCtl *ctl, int *abort)
+                 int32_t *colmap, int32_t *map_dst, int32_t *map_src, PanelCtl *ctl, int *abort, int KC, int FAR,
+                 int32_t *stage)
 {
     TimedLaunch tl(ctx, "elim_panel");
     const size_t shm = 0;
     if (mp.small)
         k_panel<RPT, NBW, true><<<1, PT, shm, ctx->stream>>>(W, ldw, m, kr0, c0, w, mp, ple, invtab, ipiv, pivval,
-                                                            pivinv, pivcol, colmap, map_dst, map_src, ctl, abort);
+                                                            pivinv, pivcol, colmap, map_dst, map_src, ctl, abort,
+                                                            KC, FAR, stage);
     else
         k_panel<RPT, NBW, false><<<1, PT, shm, ctx->stream>>>(W, ldw, m, kr0, c0, w, mp, ple, invtab, ipiv, pivval,
-                                                             pivinv, pivcol, colmap, map_dst, map_src, ctl, abort);
+                                                             pivinv, pivcol, colmap, map_dst, map_src, ctl, abort,
+                                                             KC, FAR, stage);
     return hip_check(ctx, hipGetLastError(), "k_panel launch");
 }
 
@@ -793,7 +931,8 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
     if (!W) return st;
     // pivot arrays, per-panel control blocks and row gather maps of one outer block
     const size_t npv = (size_t)mn + 64;
-    const size_t piv_bytes = sizeof(int32_t) * (4 * npv + 64 + 2 * MAXPAN * 64) + sizeof(PanelCtl) * MAXPAN + 64;
+    const size_t piv_bytes = sizeof(int32_t) * (4 * npv + 64 + 2 * MAXPAN * 64) + sizeof(PanelCtl) * MAXPAN + 256 +
+                             sizeof(int32_t) * 64 * NBO;
     int32_t *piv = (int32_t *)ws_get(ctx, WS_PIV, piv_bytes, &st);
     if (!piv) return st;
     int32_t *ipiv = piv, *pivval = piv + npv, *pivinv = piv + 2 * npv, *pivcol = piv + 3 * npv;
@@ -801,6 +940,7 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
     int32_t *bmap_dst = colmap + 64, *bmap_src = bmap_dst + MAXPAN * 64;
     PanelCtl *bctl = (PanelCtl *)(bmap_src + MAXPAN * 64);
     int *flag = (int *)(bctl + MAXPAN);
+    int32_t *stage = (int32_t *)(flag + 64);  // moved rows' old values, 64 x NBO
     int32_t *Ubig = (int32_t *)ws_get(ctx, WS_UBUF, sizeof(int32_t) * (size_t)NBO * (size_t)(n + 1), &st);
     if (!Ubig) return st;
     PanelCtl *hctl = (PanelCtl *)pinned_get(ctx, sizeof(PanelCtl) * (MAXPAN + 1) + 64, &st);
@@ -828,7 +968,8 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
     const int big_form = form == MODP_FORM_AUTO ? choose_form(m, n, NBO, p) : form;
 
     // panel t of the current outer block at (rj, cj), width w
-    auto launch_panel_t = [&](int RPT, int NBW, int t, int64_t rj, int64_t cj, int64_t w, int *abort) -> int {
+    auto launch_panel_t = [&](int RPT, int NBW, int t, int64_t KC, int64_t FAR, int64_t rj, int64_t cj, int64_t w,
+                              int *abort) -> int {
         if (t >= MAXPAN) return set_error(ctx, MODP_ERR_DEVICE, "more than %d panels in an outer block", MAXPAN);
         int32_t *mdst = bmap_dst + t * 64, *msrc = bmap_src + t * 64;
         PanelCtl *pctl = bctl + t;
@@ -836,7 +977,7 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
 #define LAUNCH_PANEL(R_, W_) \
     case R_ * 100 + W_: \
         return launch_panel<R_, W_>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, invtab, ipiv, pivval, \
-                                    pivinv, pivcol, colmap, mdst, msrc, pctl, abort);
+                                    pivinv, pivcol, colmap, mdst, msrc, pctl, abort, (int)KC, (int)FAR, stage);
         switch (key) {
             LAUNCH_PANEL(1, 16)
             LAUNCH_PANEL(1, 32)
@@ -853,21 +994,30 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
         }
 #undef LAUNCH_PANEL
     };
-    // the rest of the outer block after panel t: swaps, solve right of the
-    // panel, then G1 = the rank-npiv update of those columns for every row below
-    auto finish_t = [&](int t, int64_t KC, int64_t FAR, int64_t rj, int64_t cj, int64_t w, int64_t npiv,
-                        const int *skip) -> int {
-        const int64_t nblk = (FAR - KC) - w;
-        if (nblk > 0) {
-            TimedLaunch tl(ctx, "elim_finish");
-            k_finish<<<grid_for(nblk * 64, 4096), 256, 0, ctx->stream>>>(
-                W, ldw, KC, FAR, (int)rj, (int)cj, (int)w, (int)KC, bctl + t, pivinv, pivcol, colmap,
-                bmap_dst + t * 64, bmap_src + t * 64, Ubig, NBO, mp, skip);
-            HIP_TRY(ctx, hipGetLastError());
+    // the rest of the outer block after panel t: its row moves, and right
+    // of the panel the rows of U and the rank-npiv update below the pivots
+    auto update_t = [&](int t, int64_t KC, int64_t FAR, int64_t rj, int64_t cj, int64_t w, const int *skip) -> int {
+        const int64_t nl = cj - KC, nr = FAR - (cj + w);
+        const unsigned gy = (unsigned)((nl + UT_C - 1) / UT_C + (nr + UT_C - 1) / UT_C);
+        if (gy == 0) return MODP_OK;
+        const dim3 grid((unsigned)((m - rj + UT_R - 1) / UT_R), gy);
+        TimedLaunch tl(ctx, "elim_update");
+#define LAUNCH_UPDATE(WM_) \
+    if (mp.small) \
+        k_update<WM_, true><<<grid, 256, 0, ctx->stream>>>(W, ldw, (int)m, (int)KC, (int)FAR, (int)rj, (int)cj, (int)w, \
+                                                           bctl + t, pivinv, pivcol, bmap_dst + t * 64, stage, mp, skip); \
+    else \
+        k_update<WM_, false><<<grid, 256, 0, ctx->stream>>>(W, ldw, (int)m, (int)KC, (int)FAR, (int)rj, (int)cj, (int)w, \
+                                                            bctl + t, pivinv, pivcol, bmap_dst + t * 64, stage, mp, skip);
+        if (w <= 8) {
+            LAUNCH_UPDATE(8)
+        } else if (w <= 16) {
+            LAUNCH_UPDATE(16)
+        } else {
+            LAUNCH_UPDATE(32)
         }
-        const int64_t r1 = rj + npiv;
-        return gemm_sub(r1, m - r1, cj + w, FAR - (cj + w), cj, w, Ubig + (cj - KC) + (cj + w - KC) * NBO,
-                        MODP_FORM_PLAIN_F64, skip);
+#undef LAUNCH_UPDATE
+        return hip_check(ctx, hipGetLastError(), "k_update launch");
     };
     // end of an outer block with P pivots over `consumed` columns
     auto block_end = [&](int64_t KR, int64_t KC, int64_t FAR, int64_t P, int64_t consumed, int npanels,
@@ -920,8 +1070,8 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
             const int np = (int)pr.size();
             int64_t Pfull = 0;
             for (int t = 0; t < np; ++t) {
-                MODP_TRY(launch_panel_t(RPT, NBW, t, pr[t], pcj[t], pw[t], abort_flag));
-                MODP_TRY(finish_t(t, KC, FAR, pr[t], pcj[t], pw[t], pw[t], abort_flag));
+                MODP_TRY(launch_panel_t(RPT, NBW, t, KC, FAR, pr[t], pcj[t], pw[t], abort_flag));
+                MODP_TRY(update_t(t, KC, FAR, pr[t], pcj[t], pw[t], abort_flag));
                 Pfull += pw[t];
             }
             MODP_TRY(block_end(KR, KC, FAR, Pfull, Pfull, np, abort_flag));
@@ -943,7 +1093,7 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
                 const int64_t npiv = hctl[ts].npiv;
                 int64_t P = 0;
                 for (int t = 0; t < ts; ++t) P += pw[t];
-                if (npiv > 0) MODP_TRY(finish_t(ts, KC, FAR, pr[ts], pcj[ts], pw[ts], npiv, nullptr));
+                if (npiv > 0) MODP_TRY(update_t(ts, KC, FAR, pr[ts], pcj[ts], pw[ts], nullptr));
                 P += npiv;
                 MODP_TRY(block_end(KR, KC, FAR, P, P, ts + 1, nullptr));
                 kr += P;
@@ -959,14 +1109,14 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
                 const int64_t cj = kc, rj = kr;
                 int64_t w = FAR - cj;
                 if (w > NBW) w = NBW;
-                MODP_TRY(launch_panel_t(RPT, NBW, npanels, rj, cj, w, nullptr));
+                MODP_TRY(launch_panel_t(RPT, NBW, npanels, KC, FAR, rj, cj, w, nullptr));
                 HIP_TRY(ctx, hipMemcpyAsync(hctl, bctl + npanels, sizeof(PanelCtl), hipMemcpyDeviceToHost, ctx->stream));
                 HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
                 const PanelCtl pc = *hctl;
                 if (pc.status == 3)
                     return set_error(ctx, MODP_ERR_NOT_INVERTIBLE,
                                      "pivot has no inverse modulo %lld (modulus not prime)", (long long)p);
-                if (pc.npiv > 0) MODP_TRY(finish_t(npanels, KC, FAR, rj, cj, w, pc.npiv, nullptr));
+                if (pc.npiv > 0) MODP_TRY(update_t(npanels, KC, FAR, rj, cj, w, nullptr));
                 ++npanels;
                 kr += pc.npiv;
                 kc += pc.ncols;
