@@ -74,12 +74,13 @@ template <int VW> __device__ __forceinline__ void load_rows(const int32_t *p, in
     }
 }
 
-template <int VW> __device__ __forceinline__ void store_rows(int32_t *p, const int32_t *x)
+// VW consecutive rows of one column; x[e * stride] is row e
+template <int VW> __device__ __forceinline__ void store_rows(int32_t *p, const int32_t *x, int stride)
 {
     if constexpr (VW == 4) {
-        *reinterpret_cast<v4i_e *>(p) = v4i_e{x[0], x[1], x[2], x[3]};
+        *reinterpret_cast<v4i_e *>(p) = v4i_e{x[0], x[stride], x[2 * stride], x[3 * stride]};
     } else if constexpr (VW == 2) {
-        *reinterpret_cast<v2i_e *>(p) = v2i_e{x[0], x[1]};
+        *reinterpret_cast<v2i_e *>(p) = v2i_e{x[0], x[stride]};
     } else {
         p[0] = x[0];
     }
@@ -252,7 +253,9 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
 #pragma unroll
             for (int e = 0; e < VW; ++e) {
                 const int i = rowi(g * VW + e);
-                v[g * VW + e][j] = (i >= 0 && i < rows) ? (double)x[e] : 0.0;
+                // rows above kr0 keep their (final) values: the in-place
+                // stores at the end write them back unchanged
+                v[g * VW + e][j] = i < rows ? (double)x[e] : 0.0;
             }
         }
     }
@@ -441,47 +444,53 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
         }
     }
 
-    // every row to its final position; pivot rows take 1 at their pivot
-    // column and the normalised U row after it.  A vector of VW rows none of
-    // which moved is stored with one vector store per column; other rows one
-    // by one.
+    // Every row to its final position, in two passes: first every row is
+    // stored in place with one vector store per VW rows and column (no
+    // per-row branches), then, once those stores are complete, the rows
+    // that moved and the pivot rows (1 at the pivot column, the normalised
+    // U row after it) overwrite their destinations.  The rows that moved
+    // leave exactly the positions that other moved rows fill, so every
+    // in-place value that is wrong is overwritten.
+    {
+        // the values are converted first and the stores issued back to back
+        // (full-width panels need no per-column test)
+        const int64_t basev = (int64_t)kra + VW * tid;
+        int32_t out[RPT][NBW];
 #pragma unroll
-    for (int g = 0; g < RPT / VW; ++g) {
-        bool still = true;
+        for (int q = 0; q < RPT; ++q)
 #pragma unroll
-        for (int e = 0; e < VW; ++e) {
-            const int q = g * VW + e, i = rowi(q);
-            still = still && i >= 0 && i < rows && pos[q] == i && !((pivmask >> q) & 1u);
+            for (int j = 0; j < NBW; ++j) out[q][j] = (int32_t)(uint32_t)v[q][j];
+#pragma unroll
+        for (int g = 0; g < RPT / VW; ++g) {
+            const int64_t base = basev + g * (VW * PT);
+            int32_t *pg = W + base + (int64_t)c0 * ldw;
+            if (base + VW <= (int64_t)m) {
+                if (w == NBW) {
+#pragma unroll
+                    for (int j = 0; j < NBW; ++j) store_rows<VW>(pg + (int64_t)j * ldw, &out[g * VW][j], NBW);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < NBW; ++j)
+                        if (j < w) store_rows<VW>(pg + (int64_t)j * ldw, &out[g * VW][j], NBW);
+                }
+            }
         }
-        if (still) {
-            const int64_t base = (int64_t)kr0 + rowi(g * VW);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int i = rowi(q);
+        const bool pv = (pivmask >> q) & 1u;
+        const bool tail = i >= 0 && i < rows && kra + VW * tid + (q / VW) * (VW * PT) + VW > m;
+        if (i >= 0 && i < rows && (pv || pos[q] != i || tail)) {
+            const int64_t dst = (int64_t)kr0 + pos[q];
+            const int k = pos[q], jk = pv ? s_pcol[k] : NBW;
 #pragma unroll
             for (int j = 0; j < NBW; ++j)
-                if (j < w) {
-                    int32_t x[VW];
-#pragma unroll
-                    for (int e = 0; e < VW; ++e) x[e] = (int32_t)(uint32_t)v[g * VW + e][j];
-                    store_rows<VW>(W + base + (int64_t)(c0 + j) * ldw, x);
-                }
-            continue;
-        }
-#pragma unroll
-        for (int e = 0; e < VW; ++e) {
-            const int q = g * VW + e, i = rowi(q);
-            if (i < 0 || i >= rows) continue;
-            const int dst = kr0 + pos[q];
-            if ((pivmask >> q) & 1u) {
-                const int k = pos[q], jk = s_pcol[k];
-#pragma unroll
-                for (int j = 0; j < NBW; ++j)
-                    if (j < w)
-                        W[dst + (int64_t)(c0 + j) * ldw] =
-                            (int32_t)(j < jk ? (uint32_t)v[q][j] : (j == jk ? 1u : s_U[k][j]));
-            } else {
-#pragma unroll
-                for (int j = 0; j < NBW; ++j)
-                    if (j < w) W[dst + (int64_t)(c0 + j) * ldw] = (int32_t)(uint32_t)v[q][j];
-            }
+                if (j < w)
+                    W[dst + (int64_t)(c0 + j) * ldw] =
+                        (int32_t)(j < jk ? (uint32_t)v[q][j] : (j == jk ? 1u : s_U[k][j]));
         }
     }
 #pragma unroll
