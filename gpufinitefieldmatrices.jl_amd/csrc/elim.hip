@@ -679,6 +679,7 @@ constexpr int MAXPAN = 64;  // inner panels per outer block (NBO / smallest pane
 // swaps, then the solve with the block's L11 (P x P, in LDS; lanes hold rows
 // i and i+64), and their U rows go to Ubig (rows 0..consumed-1 indexed by
 // block column, zero for block columns without a pivot).
+template <bool LAZY>
 __global__ __launch_bounds__(1024) void k_block_finish(int32_t *__restrict__ W, int64_t ldw, int64_t n, int KR,
                                                        int KC, int FAR, int P, int consumed, int npanels,
                                                        const int32_t *__restrict__ bmap_dst,
@@ -693,17 +694,32 @@ __global__ __launch_bounds__(1024) void k_block_finish(int32_t *__restrict__ W, 
     __shared__ uint32_t Ls[128][129];
     __shared__ uint32_t invs[128];
     __shared__ int ucolmap[128];
+    __shared__ int s_pc[128];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int idx = tid; idx < P * P; idx += 1024) {
-        const int i = idx / P, j = idx % P;
-        Ls[i][j] = j < i ? (uint32_t)W[(int64_t)(KR + i) + (int64_t)pivcol[KR + j] * ldw] : 0u;
+    if (tid < 128) {
+        const int t = min(tid, P - 1);
+        s_pc[tid] = pivcol[KR + t];
+        invs[tid] = (uint32_t)pivinv[KR + t];
+        ucolmap[tid] = -1;
     }
-    if (tid < 128) ucolmap[tid] = -1;
     __syncthreads();
-    if (tid < P) {
-        invs[tid] = (uint32_t)pivinv[KR + tid];
-        ucolmap[pivcol[KR + tid] - KC] = tid;
+    // L11 into LDS: lanes take consecutive rows of one pivot column, and
+    // every load is issued before the first LDS store
+    {
+        constexpr int LPT = 128 * 128 / 1024;
+        uint32_t lv[LPT];
+#pragma unroll
+        for (int k = 0; k < LPT; ++k) {
+            const int idx = tid + 1024 * k, i = min(idx % 128, P - 1), j = min(idx / 128, P - 1);
+            lv[k] = (uint32_t)W[(int64_t)(KR + i) + (int64_t)s_pc[j] * ldw];
+        }
+#pragma unroll
+        for (int k = 0; k < LPT; ++k) {
+            const int idx = tid + 1024 * k, i = idx % 128, j = idx / 128;
+            Ls[i][j] = (j < i && i < P) ? lv[k] : 0u;
+        }
     }
+    if (tid < P) ucolmap[s_pc[tid] - KC] = tid;
     __syncthreads();
     const int64_t nout = n - (FAR - KC);
     const int um0 = lane < consumed ? ucolmap[lane] : -1;
@@ -721,29 +737,42 @@ __global__ __launch_bounds__(1024) void k_block_finish(int32_t *__restrict__ W, 
             __threadfence_block();
         }
         if (c < KC) continue;
-        uint32_t x0 = lane < P ? (uint32_t)col[KR + lane] : 0u;
-        uint32_t x1 = lane + 64 < P ? (uint32_t)col[KR + lane + 64] : 0u;
+        // forward substitution, lanes hold rows lane and lane + 64: at step
+        // j every lane computes the normalised value of row j from its own
+        // sum (only lane j's is used), readlane broadcasts it, and the rows
+        // below subtract L[i][j] x_j.  LAZY (p < 2^16): the sums of at most
+        // 127 products below 2^32 stay exact in FP64 and are reduced once.
+        double a0 = lane < P ? (double)col[KR + lane] : 0.0;
+        double a1 = lane + 64 < P ? (double)col[KR + lane + 64] : 0.0;
         for (int j = 0; j < P; ++j) {
-            uint32_t xj;
+            const double aj = j < 64 ? a0 : a1;
+            const uint32_t xl = mulmod((uint32_t)(LAZY ? dmod_near(aj, mp) : aj), invs[j], mp);
+            const uint32_t xj = (uint32_t)__builtin_amdgcn_readlane((int)xl, j & 63);
+            const double xd = (double)xj;
             if (j < 64) {
-                if (lane == j) x0 = mulmod(x0, invs[j], mp);
-                xj = (uint32_t)__shfl((int)x0, j);
-                if (lane > j && lane < P) x0 = submod(x0, mulmod(Ls[lane][j], xj, mp), mp);
-            } else {
-                if (lane == j - 64) x1 = mulmod(x1, invs[j], mp);
-                xj = (uint32_t)__shfl((int)x1, j - 64);
+                a0 = lane == j ? xd : a0;
+                if (lane > j) {
+                    a0 = fma(-(double)Ls[lane][j], xd, a0);
+                    if (!LAZY) a0 = dmod(a0, mp);
+                }
+            } else if (lane == j - 64) {
+                a1 = xd;
             }
-            if (lane + 64 > j && lane + 64 < P) x1 = submod(x1, mulmod(Ls[lane + 64][j], xj, mp), mp);
+            if (lane + 64 > j) {
+                a1 = fma(-(double)Ls[lane + 64][j], xd, a1);
+                if (!LAZY) a1 = dmod(a1, mp);
+            }
         }
+        const uint32_t x0 = (uint32_t)a0, x1 = (uint32_t)a1;
         if (lane < P) col[KR + lane] = (int32_t)x0;
         if (lane + 64 < P) col[KR + lane + 64] = (int32_t)x1;
         // Ubig rows 0..consumed-1 of this column: pivot i's value or zero
         int32_t *ucol = Ubig + (c - KC) * ldub;
-        const int a0 = um0 < 0 ? 0 : um0, a1 = um1 < 0 ? 0 : um1;
-        const uint32_t y00 = (uint32_t)__shfl((int)x0, a0 & 63), y01 = (uint32_t)__shfl((int)x1, a0 & 63);
-        const uint32_t y10 = (uint32_t)__shfl((int)x0, a1 & 63), y11 = (uint32_t)__shfl((int)x1, a1 & 63);
-        if (lane < consumed) ucol[lane] = um0 < 0 ? 0 : (int32_t)(a0 < 64 ? y00 : y01);
-        if (lane + 64 < consumed) ucol[lane + 64] = um1 < 0 ? 0 : (int32_t)(a1 < 64 ? y10 : y11);
+        const int a0i = um0 < 0 ? 0 : um0, a1i = um1 < 0 ? 0 : um1;
+        const uint32_t y00 = (uint32_t)__shfl((int)x0, a0i & 63), y01 = (uint32_t)__shfl((int)x1, a0i & 63);
+        const uint32_t y10 = (uint32_t)__shfl((int)x0, a1i & 63), y11 = (uint32_t)__shfl((int)x1, a1i & 63);
+        if (lane < consumed) ucol[lane] = um0 < 0 ? 0 : (int32_t)(a0i < 64 ? y00 : y01);
+        if (lane + 64 < consumed) ucol[lane + 64] = um1 < 0 ? 0 : (int32_t)(a1i < 64 ? y10 : y11);
     }
 }
 
@@ -1037,9 +1066,14 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
             TimedLaunch tl(ctx, "elim_block_finish");
             int64_t g = (nout + 15) / 16;
             if (g > 2048) g = 2048;
-            k_block_finish<<<(unsigned)g, 1024, 0, ctx->stream>>>(W, ldw, n, (int)KR, (int)KC, (int)FAR, (int)P,
-                                                                  (int)consumed, npanels, bmap_dst, bmap_src, bctl,
-                                                                  pivinv, pivcol, Ubig, NBO, mp, skip);
+            if (mp.small)
+                k_block_finish<true><<<(unsigned)g, 1024, 0, ctx->stream>>>(
+                    W, ldw, n, (int)KR, (int)KC, (int)FAR, (int)P, (int)consumed, npanels, bmap_dst, bmap_src, bctl,
+                    pivinv, pivcol, Ubig, NBO, mp, skip);
+            else
+                k_block_finish<false><<<(unsigned)g, 1024, 0, ctx->stream>>>(
+                    W, ldw, n, (int)KR, (int)KC, (int)FAR, (int)P, (int)consumed, npanels, bmap_dst, bmap_src, bctl,
+                    pivinv, pivcol, Ubig, NBO, mp, skip);
             HIP_TRY(ctx, hipGetLastError());
         }
         const int64_t rbeg = KR + P;
