@@ -538,7 +538,7 @@ __global__ __launch_bounds__(256) void k_update(int32_t *__restrict__ W, int64_t
     __shared__ uint32_t invs[WMAX];
     __shared__ int s_pj[WMAX];              // panel column of pivot i
     __shared__ int s_dst[64];               // row-map destinations
-    __shared__ uint32_t Ut[WMAX][UT_C];      // U rows by panel column (0: no pivot)
+    __shared__ double Ut[UT_C][WMAX];        // U by column, then panel column (0: no pivot)
     __shared__ int s_e[UT_R];               // tile row -> row-map entry, or -1
     const int tid = threadIdx.x, cl = tid >> 6;
     const int nl = c0 - KC, nr = FAR - (c0 + w);
@@ -590,7 +590,7 @@ __global__ __launch_bounds__(256) void k_update(int32_t *__restrict__ W, int64_t
     int32_t cv[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) cv[k] = W[rr + (int64_t)(cfirst + min(cl + 4 * k, nc - 1)) * ldw];
-    for (int idx = tid; idx < WMAX * UT_C; idx += 256) Ut[idx / UT_C][idx % UT_C] = 0u;
+    for (int idx = tid; idx < WMAX * UT_C; idx += 256) Ut[idx / WMAX][idx % WMAX] = 0.0;
     if (tid < UT_R) s_e[tid] = -1;
 #pragma unroll
     for (int k = 0; k < FPT; ++k) {
@@ -618,19 +618,26 @@ __global__ __launch_bounds__(256) void k_update(int32_t *__restrict__ W, int64_t
         Ls[i][j] = (j < i && i < npiv) ? Lf[i][s_pj[j]] : 0u;
     }
     __syncthreads();
-    // rows of U for the group's columns: thread k solves column k
+    // rows of U for the group's columns: thread k solves column k, right
+    // looking (each solved value is subtracted from the rows below at once,
+    // so the dependent chain per row is one normalisation and one FMA)
     if (tid < nc) {
+        double a[WMAX];
 #pragma unroll
-        for (int i = 0; i < WMAX; ++i) {
-            if (i < npiv) {
-                double acc = (double)xs[i];
+        for (int i = 0; i < WMAX; ++i) a[i] = (double)xs[i];
 #pragma unroll
-                for (int j = 0; j < i; ++j) {
-                    acc = fma(-(double)Ls[i][j], (double)xs[j], acc);
-                    if (!LAZY) acc = dmod(acc, mp);
+        for (int j = 0; j < WMAX; ++j) {
+            if (j < npiv) {
+                // x_j = a_j / pivot_j mod p; for small p in FP64 throughout
+                // (the reduced a_j times the inverse is below 2^32)
+                const double x = LAZY ? dmod_near(dmod_near(a[j], mp) * (double)invs[j], mp)
+                                      : (double)mulmod((uint32_t)a[j], invs[j], mp);
+                Ut[tid][s_pj[j]] = x;
+#pragma unroll
+                for (int i = j + 1; i < WMAX; ++i) {
+                    a[i] = fma(-(double)Ls[i][j], x, a[i]);
+                    if (!LAZY) a[i] = dmod(a[i], mp);
                 }
-                xs[i] = mulmod((uint32_t)dmod(acc, mp), invs[i], mp);
-                Ut[s_pj[i]][tid] = xs[i];
             }
         }
     }
@@ -643,7 +650,7 @@ __global__ __launch_bounds__(256) void k_update(int32_t *__restrict__ W, int64_t
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
             const int col = cl + 4 * k;
-            if (col < nc) W[r + (int64_t)(cfirst + col) * ldw] = (int32_t)Ut[pj][col];
+            if (col < nc) W[r + (int64_t)(cfirst + col) * ldw] = (int32_t)(uint32_t)Ut[col][pj];
         }
         return;
     }
@@ -663,7 +670,7 @@ __global__ __launch_bounds__(256) void k_update(int32_t *__restrict__ W, int64_t
             double acc = (double)cv[k];
 #pragma unroll
             for (int jj = 0; jj < WMAX; ++jj) {
-                acc = fma(-Lr[jj], (double)Ut[jj][col], acc);
+                acc = fma(-Lr[jj], Ut[col][jj], acc);
                 if (!LAZY) acc = dmod(acc, mp);
             }
             W[r + (int64_t)(cfirst + col) * ldw] = (int32_t)(uint32_t)(LAZY ? dmod_near(acc, mp) : acc);
