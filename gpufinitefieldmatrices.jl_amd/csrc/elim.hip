@@ -53,6 +53,7 @@ struct PanelCtl {
                  // 2 rows exhausted, 3 pivot not invertible
     int ncols;   // panel columns consumed
     int nmap;    // entries of the row gather map
+    int kr0;     // first row of the panel
 };
 
 constexpr int PT = 512;  // threads of the panel workgroup (8 waves, 256-VGPR budget)
@@ -199,13 +200,14 @@ __device__ __forceinline__ uint64_t wave_max(uint64_t v)
 // Ip), a row chosen as pivot is frozen with its normalised U row kept in
 // LDS, and at the end every row is written to its final position.
 template <int RPT, int NBW, bool SMALL>
-__global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t ldw, int m, int kr0, int c0,
+__global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t ldw, int m, int kr0_in, int c0,
                                               int w, ModP mp, int ple, const uint32_t *__restrict__ invtab,
                                               int32_t *__restrict__ ipiv, int32_t *__restrict__ pivval,
                                               int32_t *__restrict__ pivinv, int32_t *__restrict__ pivcol,
                                               int32_t *__restrict__ colmap, int32_t *__restrict__ map_dst,
                                               int32_t *__restrict__ map_src, PanelCtl *__restrict__ ctl,
-                                              int *__restrict__ abort, int KC, int FAR, int32_t *__restrict__ stage)
+                                              int *__restrict__ abort, int KC, int FAR, int32_t *__restrict__ stage,
+                                              int *__restrict__ cursor)
 {
     constexpr int NW = PT / 64;
     if (abort && *abort) {
@@ -215,9 +217,13 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
             ctl->status = 4;
             ctl->ncols = 0;
             ctl->nmap = 0;
+            ctl->kr0 = kr0_in;
         }
         return;
     }
+    // the first row comes from the device cursor when the previous panel's
+    // pivot count is not known to the host (lu, rref)
+    const int kr0 = cursor ? *cursor : kr0_in;
     typedef PivKey<SMALL> K;
     typedef typename K::T key_t;
     __shared__ key_t s_key[2][NW];
@@ -504,6 +510,8 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
         ctl->status = status;
         ctl->ncols = consumed;
         ctl->nmap = pc + s_cnt;
+        ctl->kr0 = kr0;
+        if (cursor) *cursor = kr0 + pc;
         if (abort && status != 0) *abort = 1;
     }
 }
@@ -521,7 +529,7 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
 constexpr int UT_R = 64, UT_C = 32;
 template <int WMAX, bool LAZY>
 __global__ __launch_bounds__(256) void k_update(int32_t *__restrict__ W, int64_t ldw, int m, int KC, int FAR,
-                                                int kr0, int c0, int w, const PanelCtl *__restrict__ ctl,
+                                                int c0, int w, const PanelCtl *__restrict__ ctl,
                                                 const int32_t *__restrict__ pivinv,
                                                 const int32_t *__restrict__ pivcol,
                                                 const int32_t *__restrict__ map_dst,
@@ -530,8 +538,8 @@ __global__ __launch_bounds__(256) void k_update(int32_t *__restrict__ W, int64_t
 {
     // the skip flag and the panel's counts are read together
     const int skipped = skip ? *skip : 0;
-    const int npiv = ctl->npiv, nmap = ctl->nmap;
-    if (skipped) return;
+    const int npiv = ctl->npiv, nmap = ctl->nmap, kr0 = ctl->kr0;
+    if (skipped || nmap == 0) return;
     constexpr int CPT = UT_C / 4;            // columns per thread
     __shared__ uint32_t Lf[WMAX][WMAX + 1];  // the panel's top w x w block
     __shared__ uint32_t Ls[WMAX][WMAX + 1];  // L11 (row i, pivot j < i)
@@ -558,6 +566,7 @@ __global__ __launch_bounds__(256) void k_update(int32_t *__restrict__ W, int64_t
     const int ci0 = nl + cb;        // stage column of the group's first column
     const int cfirst = c0 + w + cb;  // its column in W
     const int64_t r0 = (int64_t)kr0 + (int64_t)blockIdx.x * UT_R;
+    if (r0 >= m) return;  // the grid covers the most rows the panel can have
     const int64_t r = r0 + (tid & 63);
     const bool row_ok = r < m;
 
@@ -875,18 +884,18 @@ template <int RPT, int NBW>
 int launch_panel(modp_ctx *ctx, int32_t *W, int64_t ldw, int m, int kr0, int c0, int w, const ModP &mp, int ple,
                  const uint32_t *invtab, int32_t *ipiv, int32_t *pivval, int32_t *pivinv, int32_t *pivcol,
                  int32_t *colmap, int32_t *map_dst, int32_t *map_src, PanelCtl *ctl, int *abort, int KC, int FAR,
-                 int32_t *stage)
+                 int32_t *stage, int *cursor)
 {
     TimedLaunch tl(ctx, "elim_panel");
     const size_t shm = 0;
     if (mp.small)
         k_panel<RPT, NBW, true><<<1, PT, shm, ctx->stream>>>(W, ldw, m, kr0, c0, w, mp, ple, invtab, ipiv, pivval,
                                                             pivinv, pivcol, colmap, map_dst, map_src, ctl, abort,
-                                                            KC, FAR, stage);
+                                                            KC, FAR, stage, cursor);
     else
         k_panel<RPT, NBW, false><<<1, PT, shm, ctx->stream>>>(W, ldw, m, kr0, c0, w, mp, ple, invtab, ipiv, pivval,
                                                              pivinv, pivcol, colmap, map_dst, map_src, ctl, abort,
-                                                             KC, FAR, stage);
+                                                             KC, FAR, stage, cursor);
     return hip_check(ctx, hipGetLastError(), "k_panel launch");
 }
 
@@ -899,6 +908,8 @@ unsigned grid_for(int64_t total, int64_t cap = 16384)
 }
 
 int64_t rup(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
 
 __global__ void k_inv_table(uint32_t *__restrict__ t, ModP mp)
 {
@@ -915,10 +926,12 @@ const uint32_t *inverse_table(modp_ctx *ctx, const ModP &mp, int *st)
     const size_t need = sizeof(uint32_t) * (size_t)mp.p;
     if (ctx->invtab_cap < need) {
         if (ctx->invtab) {
-            hipStreamSynchronize(ctx->stream);
-            hipFree(ctx->invtab);
+            hipError_t e = hipStreamSynchronize(ctx->stream);
+            const hipError_t ef = hipFree(ctx->invtab);
             ctx->invtab = nullptr;
             ctx->invtab_cap = 0;
+            if (e == hipSuccess) e = ef;
+            if (e != hipSuccess) { *st = hip_check(ctx, e, "inverse table release"); return nullptr; }
         }
         hipError_t e = hipMalloc(&ctx->invtab, need);
         if (e != hipSuccess) { *st = hip_check(ctx, e, "inverse table"); return nullptr; }
@@ -986,6 +999,7 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
     PanelCtl *bctl = (PanelCtl *)(bmap_src + MAXPAN * 64);
     int *flag = (int *)(bctl + MAXPAN);
     int32_t *stage = (int32_t *)(flag + 64);  // moved rows' old values, 64 x NBO
+    int *cursor = flag + 2;                   // first row of the next panel (lu, rref)
     int32_t *Ubig = (int32_t *)ws_get(ctx, WS_UBUF, sizeof(int32_t) * (size_t)NBO * (size_t)(n + 1), &st);
     if (!Ubig) return st;
     PanelCtl *hctl = (PanelCtl *)pinned_get(ctx, sizeof(PanelCtl) * (MAXPAN + 1) + 64, &st);
@@ -1014,7 +1028,7 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
 
     // panel t of the current outer block at (rj, cj), width w
     auto launch_panel_t = [&](int RPT, int NBW, int t, int64_t KC, int64_t FAR, int64_t rj, int64_t cj, int64_t w,
-                              int *abort) -> int {
+                              int *abort, int *cursor) -> int {
         if (t >= MAXPAN) return set_error(ctx, MODP_ERR_DEVICE, "more than %d panels in an outer block", MAXPAN);
         int32_t *mdst = bmap_dst + t * 64, *msrc = bmap_src + t * 64;
         PanelCtl *pctl = bctl + t;
@@ -1022,7 +1036,7 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
 #define LAUNCH_PANEL(R_, W_) \
     case R_ * 100 + W_: \
         return launch_panel<R_, W_>(ctx, W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, invtab, ipiv, pivval, \
-                                    pivinv, pivcol, colmap, mdst, msrc, pctl, abort, (int)KC, (int)FAR, stage);
+                                    pivinv, pivcol, colmap, mdst, msrc, pctl, abort, (int)KC, (int)FAR, stage, cursor);
         switch (key) {
             LAUNCH_PANEL(1, 16)
             LAUNCH_PANEL(1, 32)
@@ -1049,10 +1063,10 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
         TimedLaunch tl(ctx, "elim_update");
 #define LAUNCH_UPDATE(WM_) \
     if (mp.small) \
-        k_update<WM_, true><<<grid, 256, 0, ctx->stream>>>(W, ldw, (int)m, (int)KC, (int)FAR, (int)rj, (int)cj, (int)w, \
+        k_update<WM_, true><<<grid, 256, 0, ctx->stream>>>(W, ldw, (int)m, (int)KC, (int)FAR, (int)cj, (int)w, \
                                                            bctl + t, pivinv, pivcol, bmap_dst + t * 64, stage, mp, skip); \
     else \
-        k_update<WM_, false><<<grid, 256, 0, ctx->stream>>>(W, ldw, (int)m, (int)KC, (int)FAR, (int)rj, (int)cj, (int)w, \
+        k_update<WM_, false><<<grid, 256, 0, ctx->stream>>>(W, ldw, (int)m, (int)KC, (int)FAR, (int)cj, (int)w, \
                                                             bctl + t, pivinv, pivcol, bmap_dst + t * 64, stage, mp, skip);
         if (w <= 8) {
             LAUNCH_UPDATE(8)
@@ -1120,7 +1134,7 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
             const int np = (int)pr.size();
             int64_t Pfull = 0;
             for (int t = 0; t < np; ++t) {
-                MODP_TRY(launch_panel_t(RPT, NBW, t, KC, FAR, pr[t], pcj[t], pw[t], abort_flag));
+                MODP_TRY(launch_panel_t(RPT, NBW, t, KC, FAR, pr[t], pcj[t], pw[t], abort_flag, nullptr));
                 MODP_TRY(update_t(t, KC, FAR, pr[t], pcj[t], pw[t], abort_flag));
                 Pfull += pw[t];
             }
@@ -1151,29 +1165,49 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
                 stop_zero = hctl[ts].status == 1;
             }
         } else if (nbo > 0) {
-            // ple: panels skip zero columns, so the next panel's row depends on
-            // the pivots found; read each panel's outcome
+            // ple: panels skip zero columns, so a panel's first row depends
+            // on the pivots found before it.  The panels of the block are
+            // enqueued at their fixed columns and each takes its first row
+            // from a device cursor that the previous one advanced; the
+            // updates read it from the panel's control block.  A panel that
+            // runs out of rows (or meets a pivot without inverse) raises the
+            // abort flag, which turns the rest of the block into no-ops, and
+            // the host reads the outcome once per block.
+            HIP_TRY(ctx, hipMemsetAsync(abort_flag, 0, sizeof(int), ctx->stream));
+            k_set_int<<<1, 1, 0, ctx->stream>>>(cursor, (int)KR);
+            HIP_TRY(ctx, hipGetLastError());
+            std::vector<int64_t> pcj, pw;
+            for (int64_t cj = KC; cj < FAR; cj += NBW) {
+                pcj.push_back(cj);
+                pw.push_back(std::min<int64_t>(NBW, FAR - cj));
+            }
+            const int np = (int)pcj.size();
+            for (int t = 0; t < np; ++t) {
+                MODP_TRY(launch_panel_t(RPT, NBW, t, KC, FAR, KR, pcj[t], pw[t], abort_flag, cursor));
+                MODP_TRY(update_t(t, KC, FAR, KR, pcj[t], pw[t], abort_flag));
+            }
+            HIP_TRY(ctx, hipMemcpyAsync(hctl, bctl, sizeof(PanelCtl) * np, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            int64_t P = 0, consumed = 0;
             int npanels = 0;
-            int64_t P = 0;
-            while (kc < FAR && kr < m) {
-                const int64_t cj = kc, rj = kr;
-                int64_t w = FAR - cj;
-                if (w > NBW) w = NBW;
-                MODP_TRY(launch_panel_t(RPT, NBW, npanels, KC, FAR, rj, cj, w, nullptr));
-                HIP_TRY(ctx, hipMemcpyAsync(hctl, bctl + npanels, sizeof(PanelCtl), hipMemcpyDeviceToHost, ctx->stream));
-                HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-                const PanelCtl pc = *hctl;
+            for (int t = 0; t < np; ++t) {
+                const PanelCtl &pc = hctl[t];
+                if (pc.status == 4) break;
+                ++npanels;
                 if (pc.status == 3)
                     return set_error(ctx, MODP_ERR_NOT_INVERTIBLE,
                                      "pivot has no inverse modulo %lld (modulus not prime)", (long long)p);
-                if (pc.npiv > 0) MODP_TRY(update_t(npanels, KC, FAR, rj, cj, w, nullptr));
-                ++npanels;
-                kr += pc.npiv;
-                kc += pc.ncols;
                 P += pc.npiv;
-                if (pc.status == 2) break;
+                consumed += pc.ncols;
+                if (pc.status != 0) {
+                    // the update after this panel was skipped by its own flag
+                    if (pc.npiv > 0) MODP_TRY(update_t(t, KC, FAR, KR, pcj[t], pw[t], nullptr));
+                    break;
+                }
             }
-            MODP_TRY(block_end(KR, KC, FAR, P, kc - KC, npanels, nullptr));
+            kr += P;
+            kc += consumed;
+            MODP_TRY(block_end(KR, KC, FAR, P, consumed, npanels, nullptr));
         }
         if (!ple && kr < m && kc < n && (stop_zero || nbo == 0 || kc > last)) {
             // pluq zero column at kc (rows >= kr zero): whole block zero -> tail,
