@@ -864,16 +864,27 @@ __global__ __launch_bounds__(256) void k_trsm_upper_unit(int32_t *__restrict__ W
 {
     __shared__ uint32_t Ts[64][65];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-        const int i = idx % nb, j = idx / nb;
-        Ts[i][j] = (uint32_t)G[i + j * ldg];
+    {
+        // every load issued before the first LDS store
+        constexpr int TPT = 64 * 64 / 256;
+        int32_t tv[TPT];
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) {
+            const int idx = tid + 256 * k, i = min(idx % 64, nb - 1), j = min(idx / 64, nb - 1);
+            tv[k] = G[i + j * ldg];
+        }
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) {
+            const int idx = tid + 256 * k, i = idx % 64, j = idx / 64;
+            Ts[i][j] = (i < nb && j < nb) ? (uint32_t)tv[k] : 0u;
+        }
     }
     __syncthreads();
     for (int64_t c = cs + (int64_t)blockIdx.x * 4 + wave; c < n; c += (int64_t)gridDim.x * 4) {
         int32_t *col = W + c * ldw + i0;
         uint32_t x = lane < nb ? (uint32_t)col[lane] : 0u;
         for (int j = nb - 1; j > 0; --j) {
-            const uint32_t xj = (uint32_t)__shfl((int)x, j);
+            const uint32_t xj = (uint32_t)__builtin_amdgcn_readlane((int)x, j);
             if (lane < j) x = submod(x, mulmod(Ts[lane][j], xj, mp), mp);
         }
         if (lane < nb) col[lane] = (int32_t)x;
@@ -1306,10 +1317,9 @@ int elim_rref(modp_ctx *ctx, int form, int kind, int64_t m, int64_t n, const voi
         for (int64_t i1 = rank; i1 > 0;) {
             const int64_t i0 = i1 > 64 ? i1 - 64 : 0;
             const int nb = (int)(i1 - i0);
-            std::vector<int32_t> pc0(1);
-            HIP_TRY(ctx, hipMemcpyAsync(pc0.data(), pivcol + i0, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            const int64_t cs = pc0[0];
+            // first pivot column of the block (the factorisation returned
+            // the pivot columns, 1-based)
+            const int64_t cs = res.pivcols[(size_t)i0] - 1;
             // T = W[i0:i1, pivcol[i0:i1]] -> G (nb x nb), then reduce the block
             k_gather_cols<<<grid_for((int64_t)nb * nb), 256, 0, ctx->stream>>>(W, ldw, i0, nb, pivcol, (int)i0, nb, G, 64);
             HIP_TRY(ctx, hipGetLastError());
