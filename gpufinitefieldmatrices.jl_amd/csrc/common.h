@@ -180,7 +180,7 @@ struct TuneEntry {
     double ms = 0.0;    // measured time of the winner
 };
 
-constexpr int WS_COUNT = 19;
+constexpr int WS_COUNT = 20;
 
 struct modp_ctx {
     int device = 0;
@@ -227,7 +227,8 @@ enum {
     WS_TRI_X = 15,     //   int32 inverse
     WS_TRI_X2 = 16,    //   second inverse (matrix inverse: L^{-1})
     WS_INV_U = 17,     // matrix inverse: PLUQ factors
-    WS_INV_L = 18
+    WS_INV_L = 18,
+    WS_PERM = 19       // lines moved by a permutation
 };
 
 // error plumbing: every internal function returns a modp_status

@@ -1,5 +1,8 @@
 // util.hip -- elementwise kernels: mod, seeded rand, eye, storage conversion,
 // transposition-stack permutations, two-modulus Chinese remaindering.
+#include <algorithm>
+#include <vector>
+
 #include "common.h"
 
 namespace {
@@ -114,6 +117,31 @@ __global__ void k_perm_cols(int64_t rows, int64_t cols, T *X, int64_t ldx, const
     }
 }
 
+// Lines (columns, or rows when rowwise) moved by a permutation, through a
+// compact buffer: first every source line is copied out, then written to its
+// destination, so the moves may overlap in any order.  pairs[2k] is the
+// destination line, pairs[2k+1] the source; len elements per line.
+template <typename T>
+__global__ void k_lines_out(int rowwise, int64_t len, const T *__restrict__ X, int64_t ldx,
+                            const int64_t *__restrict__ pairs, int64_t nm, T *__restrict__ tmp)
+{
+    const int64_t total = nm * len;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = e / len, t = e % len, src = pairs[2 * k + 1];
+        tmp[e] = rowwise ? X[src + t * ldx] : X[t + src * ldx];
+    }
+}
+template <typename T>
+__global__ void k_lines_in(int rowwise, int64_t len, T *__restrict__ X, int64_t ldx,
+                           const int64_t *__restrict__ pairs, int64_t nm, const T *__restrict__ tmp)
+{
+    const int64_t total = nm * len;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = e / len, t = e % len, dst = pairs[2 * k];
+        (rowwise ? X[dst + t * ldx] : X[t + dst * ldx]) = tmp[e];
+    }
+}
+
 unsigned grid_for(int64_t total)
 {
     int64_t b = (total + kThreads - 1) / kThreads;
@@ -222,6 +250,49 @@ int util_perm(modp_ctx *ctx, int kind, int rowwise, int64_t rows, int64_t cols, 
             return set_error(ctx, MODP_ERR_ARG, "permutation index %lld out of range 1..%lld",
                              (long long)stack_host[s], (long long)lim);
     int st = MODP_OK;
+    if (nswaps > 8) {
+        // many swaps: compose them on the host (the line that ends at j came
+        // from idx[j]) and move each displaced line once
+        std::vector<int64_t> idx((size_t)lim);
+        for (int64_t i = 0; i < lim; ++i) idx[(size_t)i] = i;
+        for (int64_t s = 0; s < nswaps; ++s) {
+            const int64_t e = inverse ? nswaps - 1 - s : s;
+            std::swap(idx[(size_t)(stack_host[2 * e] - 1)], idx[(size_t)(stack_host[2 * e + 1] - 1)]);
+        }
+        std::vector<int64_t> pairs;
+        for (int64_t j = 0; j < lim; ++j)
+            if (idx[(size_t)j] != j) {
+                pairs.push_back(j);
+                pairs.push_back(idx[(size_t)j]);
+            }
+        const int64_t nm = (int64_t)pairs.size() / 2;
+        if (nm == 0) return MODP_OK;
+        const int64_t len = rowwise ? cols : rows;
+        const size_t es = kind == EK_F64 ? 8 : 4;
+        int64_t *dpairs = (int64_t *)ws_get(ctx, WS_MISC, sizeof(int64_t) * 2 * (size_t)nm, &st);
+        if (!dpairs) return st;
+        void *tmp = ws_get(ctx, WS_PERM, es * (size_t)nm * (size_t)len, &st);
+        if (!tmp) return st;
+        HIP_TRY(ctx, hipMemcpyAsync(dpairs, pairs.data(), sizeof(int64_t) * 2 * (size_t)nm, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        const unsigned g = grid_for(nm * len);
+#define LINES(T)                                                                                              \
+    k_lines_out<T><<<g, kThreads, 0, ctx->stream>>>(rowwise, len, (const T *)X, ldx, dpairs, nm, (T *)tmp); \
+    k_lines_in<T><<<g, kThreads, 0, ctx->stream>>>(rowwise, len, (T *)X, ldx, dpairs, nm, (const T *)tmp);
+        if (kind == EK_F64) {
+            LINES(double)
+        } else if (kind == EK_F32) {
+            LINES(float)
+        } else {
+            LINES(int32_t)
+        }
+#undef LINES
+        HIP_TRY(ctx, hipGetLastError());
+        // the pair buffer is reused by the next call and the host vector
+        // must outlive the copy: keep the host program order
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return MODP_OK;
+    }
     int64_t *dstack = (int64_t *)ws_get(ctx, WS_MISC, sizeof(int64_t) * 2 * nswaps, &st);
     if (!dstack) return st;
     HIP_TRY(ctx, hipMemcpyAsync(dstack, stack_host, sizeof(int64_t) * 2 * nswaps, hipMemcpyHostToDevice,

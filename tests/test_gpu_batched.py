@@ -29,6 +29,28 @@ def test_permutations_reference_fixture(gffm):
     assert g.Array(A).astype(int).tolist() == f["A"]
 
 
+@pytest.mark.parametrize("et", [np.float64, np.float32])
+@pytest.mark.parametrize("nswaps", [3, 9, 200, 1000])
+def test_long_swap_stacks_match_oracle(gffm, nswaps, et):
+    # stacks longer than a few swaps are composed into one permutation on the
+    # host: repeated indices, self-swaps and cycles must give the sequential
+    # result, for rows and columns, forwards and inverse
+    g = gffm
+    p = 251
+    rng = np.random.default_rng(nswaps)
+    Ah = rng.integers(0, p, size=(90, 70))
+    for rowwise, lim in ((True, 90), (False, 70)):
+        st = [(int(a), int(b)) for a, b in rng.integers(1, lim + 1, size=(nswaps, 2))]
+        st[0] = (st[0][0], st[0][0])
+        fwd = O.apply_row_perm if rowwise else O.apply_col_perm
+        inv = O.apply_row_inv_perm if rowwise else O.apply_col_inv_perm
+        for f_dev, f_ref in ((g.apply_row_perm_ if rowwise else g.apply_col_perm_, fwd),
+                             (g.apply_row_inv_perm_ if rowwise else g.apply_col_inv_perm_, inv)):
+            A = g.CuModMatrix(Ah, p, elem_type=et)
+            f_dev(st, A)
+            assert np.array_equal(g.Array(A).astype(np.int64), f_ref(st, Ah))
+
+
 def test_pluq_identity_through_permutations(gffm):
     # P*A*Q == L*U using the device permutation kernels and the device GEMM
     g = gffm
