@@ -107,6 +107,33 @@ def test_pluq_lu_tall_vs_oracle(gffm, shape, r):
     assert np.array_equal(g.Array(L).astype(np.int64), ref["L"])
 
 
+@pytest.mark.parametrize("p", [8388593, 2 ** 26 - 5])
+@pytest.mark.parametrize("kind", ["full", "deficient", "zero_cols"])
+def test_medium_large_prime_all_modes(gffm, p, kind):
+    # panels, block updates and block ends on the reduce-every-step path
+    # (p >= 2^16), several outer blocks, every elimination mode
+    g = gffm
+    m, n = 600, 520
+    print(f"large-p medium kind={kind} p={p}", flush=True)
+    if kind == "full":
+        Ah = O.draw_matrix(m, n, p, seed=21)
+    elif kind == "deficient":
+        Ah = rank_deficient(m, n, 300, p, seed=21)
+    else:
+        Ah = O.draw_matrix(m, n, p, seed=21)
+        Ah[:, [5, 130, 131, 300]] = 0
+    check_pluq_equal(g, Ah, p, form=0)
+    U, L, perm, piv, rank = g.lu_full(dev(g, Ah, p))
+    ref = O.lu(Ah, p)
+    assert rank == ref["rank"] and perm == ref["perm"].tolist() and piv == ref["pivcols"].tolist()
+    assert np.array_equal(g.Array(U).astype(np.int64), ref["U"])
+    assert np.array_equal(g.Array(L).astype(np.int64), ref["L"])
+    R, piv, rank = g.rref_full(dev(g, Ah, p))
+    ref = O.rref(Ah, p)
+    assert rank == ref["rank"] and piv == ref["pivcols"].tolist()
+    assert np.array_equal(g.Array(R).astype(np.int64), ref["R"])
+
+
 def test_pluq_float32_storage(gffm):
     for p in (7, 251, 65521):
         Ah = O.draw_matrix(60, 45, p, seed=4)
