@@ -208,6 +208,11 @@ int modp_ctx_destroy(modp_ctx *ctx)
     for (auto e : ctx->event_pool) hipEventDestroy(e);
     for (auto e : ctx->slots)
         if (e) hipEventDestroy(e);
+    for (int i = 0; i < modp_ctx::NSIDE; ++i) {
+        if (ctx->side[i]) hipStreamDestroy(ctx->side[i]);
+        if (ctx->join_ev[i]) hipEventDestroy(ctx->join_ev[i]);
+    }
+    if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return MODP_OK;

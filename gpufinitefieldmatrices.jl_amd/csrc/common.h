@@ -205,6 +205,12 @@ struct modp_ctx {
     // autotuner: MODP_TUNE_OFF / _CACHED / _ON_MISS and its cache
     int tune_mode = 0;
     std::map<TuneKey, TuneEntry> tune_cache;
+    // side streams (default priority) and fork/join events for independent
+    // small launches; created on first use
+    static constexpr int NSIDE = 8;
+    hipStream_t side[NSIDE] = {nullptr};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[NSIDE] = {nullptr};
 };
 
 // workspace slots

@@ -1,5 +1,6 @@
-// trinv.hip -- inverses mod p: triangular matrices (divide and conquer on
-// 64-row diagonal blocks, off-diagonal blocks by GEMM) and square matrices
+// trinv.hip -- inverses mod p: triangular matrices (64-row diagonal blocks
+// inverted in LDS, then the off-diagonal blocks by GEMM, level by level with
+// the independent products of a level on side streams) and square matrices
 // through PLUQ, A^{-1} = Q U^{-1} L^{-1} P.
 //
 // Reference: upper/lower_triangular_inverse_no_copy
@@ -9,6 +10,9 @@
 // (src/CuModMatrix/CuModMatrix.jl:351-502), which multiplies the permuted
 // triangular inverses.  Inverses are unique, so the split and the GEMM form
 // do not change the result.
+#include <algorithm>
+#include <vector>
+
 #include "common.h"
 
 namespace {
@@ -78,31 +82,84 @@ GemmArgs i32_gemm(int64_t m, int64_t n, int64_t k, const int32_t *A, int64_t lda
     return g;
 }
 
-// Off-diagonal blocks of X = T^{-1} on rows/columns [lo, lo + n), after the
-// diagonal blocks are in place and X is zero elsewhere.  Split at h (a
-// multiple of TB): upper X12 = -X11 (T12 X22), lower X21 = -X22 (T21 X11).
-int trinv_rec(modp_ctx *ctx, bool upper, const int32_t *T, int64_t ldt, int32_t *X, int64_t ldx, int32_t *tmp,
-              int64_t lo, int64_t n, const ModP &mp)
+// Off-diagonal blocks of X = T^{-1}, bottom-up by levels, after the diagonal
+// blocks are in place and X is zero elsewhere.  At level b the blocks
+// [lo, lo + 2b) (lo a multiple of 2b) are completed from their halves:
+// upper X12 = -X11 (T12 X22), lower X21 = -X22 (T21 X11), with X11 and X22
+// finished at the level below.  The pairs of one level are independent:
+// while they are small (b < 512) they run on the side streams, each pair's
+// two products on one stream in the plain form without K slices (no shared
+// workspace), with a fork and a join per level; larger levels run on the
+// library stream with the automatic form.
+int set_stream(modp_ctx *ctx, hipStream_t s)
 {
-    if (n <= TB) return MODP_OK;
-    const int64_t nbk = (n + TB - 1) / TB;
-    const int64_t h = (nbk / 2) * TB, r = n - h;
-    MODP_TRY(trinv_rec(ctx, upper, T, ldt, X, ldx, tmp, lo, h, mp));
-    MODP_TRY(trinv_rec(ctx, upper, T, ldt, X, ldx, tmp, lo + h, r, mp));
+    ctx->stream = s;
+    return MODP_OK;
+}
+
+int side_streams(modp_ctx *ctx)
+{
+    if (ctx->fork_ev) return MODP_OK;
+    for (int i = 0; i < modp_ctx::NSIDE; ++i) {
+        HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking));
+        HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
+    }
+    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    return MODP_OK;
+}
+
+int trinv_pair(modp_ctx *ctx, bool upper, const int32_t *T, int64_t ldt, int32_t *X, int64_t ldx, int32_t *tmp,
+               int64_t lo, int64_t h, int64_t r, const ModP &mp, bool plain)
+{
     const int32_t *T11 = T + lo + lo * ldt;
     const int32_t *X11 = X + lo + lo * ldx, *X22 = X + (lo + h) + (lo + h) * ldx;
+    GemmArgs g1, g2;
     if (upper) {
         // tmp (h x r) = T12 X22; X12 = 0 - X11 tmp
-        const GemmArgs g1 = i32_gemm(h, r, r, T11 + h * ldt, ldt, X22, ldx, tmp, h, EPI_STORE, mp);
-        MODP_TRY(gemm_run(ctx, g1, gemm_default_config(ctx, g1, MODP_FORM_AUTO)));
-        const GemmArgs g2 = i32_gemm(h, r, h, X11, ldx, tmp, h, X + lo + (lo + h) * ldx, ldx, EPI_SUB, mp);
-        MODP_TRY(gemm_run(ctx, g2, gemm_default_config(ctx, g2, MODP_FORM_AUTO)));
+        g1 = i32_gemm(h, r, r, T11 + h * ldt, ldt, X22, ldx, tmp, h, EPI_STORE, mp);
+        g2 = i32_gemm(h, r, h, X11, ldx, tmp, h, X + lo + (lo + h) * ldx, ldx, EPI_SUB, mp);
     } else {
         // tmp (r x h) = T21 X11; X21 = 0 - X22 tmp
-        const GemmArgs g1 = i32_gemm(r, h, h, T11 + h, ldt, X11, ldx, tmp, r, EPI_STORE, mp);
-        MODP_TRY(gemm_run(ctx, g1, gemm_default_config(ctx, g1, MODP_FORM_AUTO)));
-        const GemmArgs g2 = i32_gemm(r, h, r, X22, ldx, tmp, r, X + (lo + h) + lo * ldx, ldx, EPI_SUB, mp);
-        MODP_TRY(gemm_run(ctx, g2, gemm_default_config(ctx, g2, MODP_FORM_AUTO)));
+        g1 = i32_gemm(r, h, h, T11 + h, ldt, X11, ldx, tmp, r, EPI_STORE, mp);
+        g2 = i32_gemm(r, h, r, X22, ldx, tmp, r, X + (lo + h) + lo * ldx, ldx, EPI_SUB, mp);
+    }
+    for (const GemmArgs *g : {&g1, &g2}) {
+        GemmConfig cfg = gemm_default_config(ctx, *g, plain ? MODP_FORM_PLAIN_F64 : MODP_FORM_AUTO);
+        if (plain) cfg.ksplit = 1;
+        MODP_TRY(gemm_run(ctx, *g, cfg));
+    }
+    return MODP_OK;
+}
+
+int trinv_levels(modp_ctx *ctx, bool upper, const int32_t *T, int64_t ldt, int32_t *X, int64_t ldx, int32_t *tmp,
+                 int64_t k, const ModP &mp)
+{
+    MODP_TRY(side_streams(ctx));
+    hipStream_t main_stream = ctx->stream;
+    for (int64_t b = TB; b < k; b *= 2) {
+        std::vector<int64_t> los;
+        for (int64_t lo = 0; lo + b < k; lo += 2 * b) los.push_back(lo);
+        const bool par = b < 512 && los.size() > 1;
+        if (!par) {
+            for (size_t i = 0; i < los.size(); ++i)
+                MODP_TRY(trinv_pair(ctx, upper, T, ldt, X, ldx, tmp, los[i], b, std::min(b, k - los[i] - b), mp, false));
+            continue;
+        }
+        HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, main_stream));
+        for (int i = 0; i < modp_ctx::NSIDE; ++i) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0));
+        int st = MODP_OK;
+        for (size_t i = 0; i < los.size() && st == MODP_OK; ++i) {
+            set_stream(ctx, ctx->side[i % modp_ctx::NSIDE]);
+            // each pair has its own b x b slice of the temporary
+            st = trinv_pair(ctx, upper, T, ldt, X, ldx, tmp + (int64_t)i * b * b, los[i], b,
+                            std::min(b, k - los[i] - b), mp, true);
+        }
+        set_stream(ctx, main_stream);
+        for (int i = 0; i < modp_ctx::NSIDE; ++i) {
+            HIP_TRY(ctx, hipEventRecord(ctx->join_ev[i], ctx->side[i]));
+            HIP_TRY(ctx, hipStreamWaitEvent(main_stream, ctx->join_ev[i], 0));
+        }
+        MODP_TRY(st);
     }
     return MODP_OK;
 }
@@ -140,7 +197,7 @@ static int trinv_i32(modp_ctx *ctx, bool upper, const int32_t *T, int64_t ldt, i
     if (hbad)
         return set_error(ctx, MODP_ERR_NOT_INVERTIBLE, "triangular matrix has a zero (or non-invertible) diagonal entry modulo %lld",
                          (long long)mp.p);
-    return trinv_rec(ctx, upper, T, ldt, X, ldx, tmp, 0, k, mp);
+    return trinv_levels(ctx, upper, T, ldt, X, ldx, tmp, k, mp);
 }
 
 int tri_inverse(modp_ctx *ctx, int kind, int upper, int64_t m, int64_t n, const void *A, int64_t lda, void *Ainv,
