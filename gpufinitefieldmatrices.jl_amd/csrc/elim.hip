@@ -1219,6 +1219,18 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
             kr += P;
             kc += consumed;
             MODP_TRY(block_end(KR, KC, FAR, P, consumed, npanels, nullptr));
+            if (P < consumed && kr < m && kc < n) {
+                // the block met zero columns: if nothing nonzero is left
+                // below row kr, no later column has a pivot and the
+                // factorisation is complete (rank = kr)
+                HIP_TRY(ctx, hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+                k_any_nonzero<<<grid_for((m - kr) * (n - kc), 4096), 256, 0, ctx->stream>>>(W, ldw, kr, m, kc, n, flag);
+                HIP_TRY(ctx, hipGetLastError());
+                int *hflag = (int *)hctl;
+                HIP_TRY(ctx, hipMemcpyAsync(hflag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+                HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                if (!*hflag) done = true;
+            }
         }
         if (!ple && kr < m && kc < n && (stop_zero || nbo == 0 || kc > last)) {
             // pluq zero column at kc (rows >= kr zero): whole block zero -> tail,
