@@ -522,11 +522,11 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
 // and the rank-npiv update of every row below the pivots.  The old values
 // of moved rows come from the stage the panel kernel filled before anything
 // moved, so no block reads a row that another block writes.  Grid: x =
-// 64-row tiles from kr0 down, y = 16-column groups, the groups left of the
+// 64-row tiles from kr0 down, y = 8-column groups, the groups left of the
 // panel first (moves only, done by the blocks of the first row tile).
 // LAZY (p < 2^16): at most 32 products below 2^32 are summed exactly in FP64
 // before one reduction; otherwise every product is reduced.
-constexpr int UT_R = 64, UT_C = 16;
+constexpr int UT_R = 64, UT_C = 8;
 template <int WMAX, bool LAZY>
 __global__ __launch_bounds__(256) void k_update(int32_t *__restrict__ W, int64_t ldw, int m, int KC, int FAR,
                                                 int c0, int w, const PanelCtl *__restrict__ ctl,
