@@ -729,10 +729,12 @@ __global__ __launch_bounds__(1024) void k_block_finish(int32_t *__restrict__ W, 
             const int idx = tid + 1024 * k, i = min(idx % 128, P - 1), j = min(idx / 128, P - 1);
             lv[k] = (uint32_t)W[(int64_t)(KR + i) + (int64_t)s_pc[j] * ldw];
         }
+        // rows scaled by their pivot's inverse: the system becomes unit
+        // lower triangular, so the substitution needs no multiply per step
 #pragma unroll
         for (int k = 0; k < LPT; ++k) {
             const int idx = tid + 1024 * k, i = idx % 128, j = idx / 128;
-            Ls[i][j] = (j < i && i < P) ? lv[k] : 0u;
+            Ls[i][j] = (j < i && i < P) ? mulmod(lv[k], invs[i], mp) : 0u;
         }
     }
     if (tid < P) ucolmap[s_pc[tid] - KC] = tid;
@@ -753,16 +755,17 @@ __global__ __launch_bounds__(1024) void k_block_finish(int32_t *__restrict__ W, 
             __threadfence_block();
         }
         if (c < KC) continue;
-        // forward substitution, lanes hold rows lane and lane + 64: at step
-        // j every lane computes the normalised value of row j from its own
-        // sum (only lane j's is used), readlane broadcasts it, and the rows
-        // below subtract L[i][j] x_j.  LAZY (p < 2^16): the sums of at most
-        // 127 products below 2^32 stay exact in FP64 and are reduced once.
-        double a0 = lane < P ? (double)col[KR + lane] : 0.0;
-        double a1 = lane + 64 < P ? (double)col[KR + lane + 64] : 0.0;
+        // forward substitution with the unit triangular system (rows and
+        // right-hand side scaled by the pivots' inverses), lanes hold rows
+        // lane and lane + 64: at step j every lane reduces its own sum (only
+        // lane j's is used), readlane broadcasts it, and the rows below
+        // subtract L[i][j] x_j.  LAZY (p < 2^16): the sums of at most 127
+        // products below 2^32 stay exact in FP64 and are reduced once.
+        double a0 = lane < P ? (double)mulmod((uint32_t)col[KR + lane], invs[lane], mp) : 0.0;
+        double a1 = lane + 64 < P ? (double)mulmod((uint32_t)col[KR + lane + 64], invs[lane + 64], mp) : 0.0;
         for (int j = 0; j < P; ++j) {
             const double aj = j < 64 ? a0 : a1;
-            const uint32_t xl = mulmod((uint32_t)(LAZY ? dmod_near(aj, mp) : aj), invs[j], mp);
+            const uint32_t xl = (uint32_t)(LAZY ? dmod_near(aj, mp) : aj);
             const uint32_t xj = (uint32_t)__builtin_amdgcn_readlane((int)xl, j & 63);
             const double xd = (double)xj;
             if (j < 64) {
