@@ -1157,10 +1157,11 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
                 Pfull += pw[t];
             }
             MODP_TRY(block_end(KR, KC, FAR, Pfull, Pfull, np, abort_flag));
-            HIP_TRY(ctx, hipMemcpyAsync(hctl, bctl, sizeof(PanelCtl) * np, hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(ctx, hipMemcpyAsync(hctl + MAXPAN, abort_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            // the control blocks and, right after them, the flags: one copy
+            HIP_TRY(ctx, hipMemcpyAsync(hctl, bctl, sizeof(PanelCtl) * MAXPAN + 2 * sizeof(int), hipMemcpyDeviceToHost,
+                                        ctx->stream));
             HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-            const int aborted = *(int *)(hctl + MAXPAN);
+            const int aborted = reinterpret_cast<const int *>(hctl + MAXPAN)[1];
             if (!aborted) {
                 kr += Pfull;
                 kc += Pfull;
@@ -1332,7 +1333,8 @@ int elim_rref(modp_ctx *ctx, int form, int kind, int64_t m, int64_t n, const voi
     if (rank > 1) {
         int32_t *G = (int32_t *)ws_get(ctx, WS_SCRATCH, sizeof(int32_t) * (size_t)(rank + 64) * 64, &st);
         if (!G) return st;
-        const int bform = form == MODP_FORM_AUTO ? MODP_FORM_PLAIN_F64 : form;
+        const int bform = form != MODP_FORM_AUTO ? form
+                          : (m >= 1024 && n >= 1024) ? MODP_FORM_LIMB_I8 : MODP_FORM_PLAIN_F64;
         for (int64_t i1 = rank; i1 > 0;) {
             const int64_t i0 = i1 > 64 ? i1 - 64 : 0;
             const int nb = (int)(i1 - i0);
