@@ -191,6 +191,17 @@ def test_not_prime_modulus_is_an_error(gffm):
     A = dev(g, [[2, 1], [1, 1]], 4)
     with pytest.raises(g.CuModMatrixModulusNotPrimeException):
         g.pluq(A)
+    # mid-panel: 20 unit pivots, then only even residues mod 4 (pivot 2 has
+    # no inverse) -- every mode raises, lu and rref from their device-cursor
+    # panels
+    rng = np.random.default_rng(8)
+    Ah = np.zeros((200, 150), dtype=np.int64)
+    Ah[:20, :20] = np.eye(20, dtype=np.int64)
+    Ah[:, 20:] = 2 * rng.integers(0, 2, size=(200, 130))
+    Ah[25, 20] = 2
+    for f in (g.pluq, g.lu_full, g.rref_full):
+        with pytest.raises(g.CuModMatrixModulusNotPrimeException):
+            f(dev(g, Ah, 4))
 
 
 @pytest.mark.parametrize("kind", ["full", "rank3000"])
