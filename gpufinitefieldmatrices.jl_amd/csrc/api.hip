@@ -41,8 +41,8 @@ void *ws_get(modp_ctx *ctx, int slot, size_t bytes, int *status)
     if (ctx->ws_size[slot] >= bytes) return ctx->ws[slot];
     if (ctx->ws[slot]) {
         // in-flight kernels may still read the old buffer
-        hipStreamSynchronize(ctx->stream);
-        hipFree(ctx->ws[slot]);
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ctx->ws[slot]);
         ctx->ws[slot] = nullptr;
         ctx->ws_size[slot] = 0;
     }
@@ -62,8 +62,8 @@ void *pinned_get(modp_ctx *ctx, size_t bytes, int *status)
 {
     if (ctx->pinned_size >= bytes) return ctx->pinned;
     if (ctx->pinned) {
-        hipStreamSynchronize(ctx->stream);
-        hipHostFree(ctx->pinned);
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipHostFree(ctx->pinned);
         ctx->pinned = nullptr;
         ctx->pinned_size = 0;
     }
@@ -87,7 +87,7 @@ static hipEvent_t take_event(modp_ctx *ctx)
         return e;
     }
     hipEvent_t e = nullptr;
-    hipEventCreate(&e);
+    (void)hipEventCreate(&e);
     return e;
 }
 
@@ -96,12 +96,12 @@ TimedLaunch::TimedLaunch(modp_ctx *c, const char *name) : ctx(c), t(nullptr)
     if (!ctx->timing) return;
     ctx->timings.push_back(KernelTiming{name, take_event(ctx), take_event(ctx)});
     t = &ctx->timings.back();
-    hipEventRecord(t->a, ctx->stream);
+    (void)hipEventRecord(t->a, ctx->stream);
 }
 
 TimedLaunch::~TimedLaunch()
 {
-    if (t) hipEventRecord(ctx->timings.back().b, ctx->stream);
+    if (t) (void)hipEventRecord(ctx->timings.back().b, ctx->stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -198,22 +198,22 @@ int modp_ctx_create(int device, modp_ctx **out)
 int modp_ctx_destroy(modp_ctx *ctx)
 {
     if (!ctx) return MODP_OK;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     for (int i = 0; i < WS_COUNT; ++i)
-        if (ctx->ws[i]) hipFree(ctx->ws[i]);
-    if (ctx->pinned) hipHostFree(ctx->pinned);
-    if (ctx->invtab) hipFree(ctx->invtab);
-    for (auto &t : ctx->timings) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
-    for (auto e : ctx->event_pool) hipEventDestroy(e);
+        if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->invtab) (void)hipFree(ctx->invtab);
+    for (auto &t : ctx->timings) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     for (auto e : ctx->slots)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     for (int i = 0; i < modp_ctx::NSIDE; ++i) {
-        if (ctx->side[i]) hipStreamDestroy(ctx->side[i]);
-        if (ctx->join_ev[i]) hipEventDestroy(ctx->join_ev[i]);
+        if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
+        if (ctx->join_ev[i]) (void)hipEventDestroy(ctx->join_ev[i]);
     }
-    if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
-    hipStreamDestroy(ctx->stream);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return MODP_OK;
 }

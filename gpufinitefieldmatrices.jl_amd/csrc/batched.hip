@@ -135,8 +135,8 @@ double plan_time(modp_ctx *ctx, modp_gemm_plan *pl, int *st)
         if (*st == MODP_OK) *st = hip_check(ctx, hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
         if (*st == MODP_OK) best = std::min(best, (double)t);
     }
-    if (a) hipEventDestroy(a);
-    if (b) hipEventDestroy(b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
     return best;
 }
 
@@ -291,12 +291,12 @@ int batched_gemm_run(modp_ctx *ctx, modp_gemm_plan *pl)
 int batched_gemm_destroy(modp_ctx *ctx, modp_gemm_plan *pl)
 {
     if (!pl) return MODP_OK;
-    hipStreamSynchronize(ctx->stream);
-    if (pl->exec) hipGraphExecDestroy(pl->exec);
-    if (pl->graph) hipGraphDestroy(pl->graph);
-    if (pl->djobs) hipFree(pl->djobs);
-    if (pl->ditems) hipFree(pl->ditems);
-    if (pl->dcounter) hipFree(pl->dcounter);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    if (pl->djobs) (void)hipFree(pl->djobs);
+    if (pl->ditems) (void)hipFree(pl->ditems);
+    if (pl->dcounter) (void)hipFree(pl->dcounter);
     delete pl;
     return MODP_OK;
 }

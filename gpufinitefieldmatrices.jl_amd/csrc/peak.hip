@@ -97,8 +97,8 @@ int probe_peak(modp_ctx *ctx, int kind, double *rate)
         if (st == MODP_OK) st = hip_check(ctx, hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
         if (st == MODP_OK && r > 0 && t < best) best = t;  // first run warms up
     }
-    if (a) hipEventDestroy(a);
-    if (b) hipEventDestroy(b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
     if (st == MODP_OK) *rate = work / (best * 1e-3);  // ops/s or bytes/s
     return st;
 }

@@ -52,8 +52,8 @@ int time_config(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg, double 
             if (t > 50.f) reps = 1;
         }
     }
-    if (a) hipEventDestroy(a);
-    if (b) hipEventDestroy(b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
     *ms = best;
     return st;
 }
