@@ -232,6 +232,9 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
     __shared__ int s_cidx[2][NW];                // candidate local row index
     __shared__ uint32_t s_U[NBW][NBW];
     __shared__ int s_pcol[NBW];
+    // per pivot: source position, value, inverse; per column: pivot or -1
+    // (kept in LDS during the steps, written out at the end)
+    __shared__ int s_ip[NBW], s_pv[NBW], s_pi[NBW], s_cmap[NBW];
     __shared__ int s_cnt;
     __shared__ int s_msrc[2 * NBW];  // source row of each row-map entry
 
@@ -354,7 +357,7 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
         const bool piv = run && V != 0 && inv != 0;
         if (run && V == 0) {
             if (ple) {
-                if (tid == 0) colmap[jc] = -1;
+                if (tid == 0) s_cmap[jc] = -1;
             } else {
                 status = 1;
                 consumed = jc;
@@ -401,12 +404,11 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
                 }
         }
         if (piv && tid == 0) {
-            ipiv[kr0 + pc] = kr0 + Ip;
-            pivval[kr0 + pc] = (int32_t)V;
-            pivinv[kr0 + pc] = (int32_t)inv;
-            pivcol[kr0 + pc] = c0 + jc;
-            colmap[jc] = pc;
+            s_ip[pc] = Ip;
+            s_pv[pc] = (int)V;
+            s_pi[pc] = (int)inv;
             s_pcol[pc] = jc;
+            s_cmap[jc] = pc;
         }
         pc += piv ? 1 : 0;
     };
@@ -504,8 +506,14 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
         const int idx = tid + k * PT;
         if (idx < nstage) stage[idx % nmap_all + 64 * (idx / nmap_all)] = sv[k];
     }
+    if (tid < pc) {
+        ipiv[kr0 + tid] = kr0 + s_ip[tid];
+        pivval[kr0 + tid] = s_pv[tid];
+        pivinv[kr0 + tid] = s_pi[tid];
+        pivcol[kr0 + tid] = c0 + s_pcol[tid];
+    }
+    if (tid < w) colmap[tid] = tid < consumed ? s_cmap[tid] : -1;
     if (tid == 0) {
-        for (int j = consumed; j < w; ++j) colmap[j] = -1;
         ctl->npiv = pc;
         ctl->status = status;
         ctl->ncols = consumed;
