@@ -257,7 +257,7 @@ def bench_pluq(g, ctx):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported",
+        print(json.dumps({"error": "the library runs on one GPU; --gpus > 1 is not supported",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     import gpufinitefieldmatrices as g

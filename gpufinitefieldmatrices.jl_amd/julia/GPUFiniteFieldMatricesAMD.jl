@@ -7,7 +7,7 @@
 # one `ccall` into the C ABI declared in include/modp.h.  No CUDA.jl.
 #
 # Julia is not installed where this library is built and tested; this file is
-# the binding a maintainer loads next to the built library (see INTEGRATION.md).
+# the binding a maintainer loads next to the built library.
 module GPUFiniteFieldMatricesAMD
 
 using LinearAlgebra
