@@ -655,7 +655,7 @@ int modp_probe_peak(modp_ctx *ctx, int kind, double *rate)
 {
     CTX_CHECK(ctx);
     if (!rate) return set_error(ctx, MODP_ERR_ARG, "null output");
-    if (kind < 0 || kind > 2) return set_error(ctx, MODP_ERR_ARG, "unknown peak kind %d", kind);
+    if (kind < 0 || kind > 3) return set_error(ctx, MODP_ERR_ARG, "unknown peak kind %d", kind);
     return probe_peak(ctx, kind, rate);
 }
 

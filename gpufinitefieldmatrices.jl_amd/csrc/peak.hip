@@ -1,10 +1,11 @@
 // peak.hip -- microbenchmarks of the device peaks that bound the hot path:
-// dense int8 MFMA (v_mfma_i32_16x16x64_i8), FP64 MFMA (v_mfma_f64_16x16x4f64)
-// and HBM streaming bandwidth.  Used by the benchmark to report roofline
+// dense int8 MFMA (v_mfma_i32_16x16x64_i8 and v_mfma_i32_32x32x32_i8), FP64
+// MFMA (v_mfma_f64_16x16x4f64) and HBM streaming bandwidth.  Used by the benchmark to report roofline
 // fractions against measured as well as published peaks.
 #include "common.h"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 namespace {
@@ -26,6 +27,26 @@ __global__ __launch_bounds__(256) void k_peak_i8(int iters, int seed, int *out)
     int s = 0;
 #pragma unroll
     for (int c = 0; c < CHAINS; ++c) s += acc[c][0] + acc[c][1] + acc[c][2] + acc[c][3];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_peak_i8_32(int iters, int seed, int *out)
+{
+    const int lane = threadIdx.x & 63;
+    v4i a = v4i{seed + lane, lane, 3, seed};
+    v4i b = v4i{lane ^ seed, 7, seed, lane};
+    v16i acc[CHAINS];
+#pragma unroll
+    for (int c = 0; c < CHAINS; ++c) acc[c] = v16i{c};
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int c = 0; c < CHAINS; ++c) acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc[c], 0, 0, 0);
+    }
+    int s = 0;
+#pragma unroll
+    for (int c = 0; c < CHAINS; ++c)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) s += acc[c][e];
     out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
@@ -81,6 +102,9 @@ int probe_peak(modp_ctx *ctx, int kind, double *rate)
         if (kind == 0) {
             k_peak_i8<<<grid, 256, 0, ctx->stream>>>(iters, r, (int *)buf);
             work = (double)grid * 4 * iters * CHAINS * (16.0 * 16 * 64 * 2);
+        } else if (kind == 3) {
+            k_peak_i8_32<<<grid, 256, 0, ctx->stream>>>(iters / 2, r, (int *)buf);
+            work = (double)grid * 4 * (iters / 2) * CHAINS * (32.0 * 32 * 32 * 2);
         } else if (kind == 1) {
             k_peak_f64<<<grid, 256, 0, ctx->stream>>>(iters / 4, 0.5 + r, (double *)buf);
             work = (double)grid * 4 * (iters / 4) * CHAINS * (16.0 * 16 * 4 * 2);

@@ -12,11 +12,12 @@ module GPUFiniteFieldMatricesAMD
 
 using LinearAlgebra
 
-export ModMatrix, CuModMatrix, pluq_gpu_kernel, pluq, lu_ple, rref, mod_inv,
+export ModMatrix, CuModMatrix, pluq_gpu_kernel, pluq, lu_ple, lu_gpu_type, rref, rref_gpu_type, mod_inv,
        apply_row_perm!, apply_row_inv_perm!, apply_col_perm!, apply_col_inv_perm!,
        stripe_mul!, find_max_stripe_ops, mat_mul_gpu_type, mat_mul_type_inplace!,
        inverse, is_invertible, is_invertible_with_inverse,
-       upper_triangular_inverse_no_copy, lower_triangular_inverse_no_copy, crt
+       upper_triangular_inverse_no_copy, lower_triangular_inverse_no_copy, crt,
+       eye, mod_elements!, change_modulus, change_modulus_no_alloc!
 
 const LIB = get(ENV, "MODP_HIP_LIB", joinpath(@__DIR__, "..", "gpufinitefieldmatrices", "libmodp_hip.so"))
 const TILE_WIDTH = 32
@@ -128,7 +129,7 @@ function ModMatrix(A::AbstractMatrix, N::Integer; mod::Bool=true, new_size=nothi
         "Modulus $N is bigger than 2^52 = $(2^52), the largest supported modulus"))
     T = elem_type
     M = ModMatrix{T}(size(A, 1), size(A, 2), N; zero=false)
-    host = zeros(T, M.padded...)
+    host = Base.zeros(T, M.padded...)
     host[1:size(A, 1), 1:size(A, 2)] .= convert.(T, A)   # errors when inexact
     check(ccall((:modp_memcpy_h2d, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Csize_t),
                 context().h, M.data.ptr, host, sizeof(host)), "upload")
@@ -147,7 +148,59 @@ function Base.Array(A::ModMatrix{T}) where {T}
     return host[1:A.size[1], 1:A.size[2]]
 end
 
+# the package's own zeros(T, rows, cols, N) (CuModMatrix.jl:530-534): a new
+# function of this module, so host arrays above use Base.zeros explicitly
 zeros(::Type{T}, r::Integer, c::Integer, N::Integer) where {T} = ModMatrix{T}(r, c, N)
+
+"""
+    eye(T, n, N)
+
+n x n identity mod N (CuModMatrix.jl:510-513).
+"""
+function eye(::Type{T}, n::Integer, N::Integer) where {T}
+    M = ModMatrix{T}(n, n, N)
+    check(ccall((:modp_eye, LIB), Cint, (Ptr{Cvoid}, Cint, Int64, Int64, Ptr{Cvoid}, Int64),
+                context().h, dtype_code(T), n, n, M.data.ptr, ld(M)), "eye")
+    return M
+end
+
+"""
+    mod_elements!(A, mod_N=-1)
+
+Reduce every stored entry (padding included) mod mod_N, or mod A.N when
+mod_N <= 0 (CuModMatrix.jl:712-716).
+"""
+function mod_elements!(A::ModMatrix{T}, mod_N::Integer=-1) where {T}
+    N = mod_N > 0 ? mod_N : A.N
+    check(ccall((:modp_mod, LIB), Cint, (Ptr{Cvoid}, Cint, Int64, Int64, Ptr{Cvoid}, Int64, Int64),
+                context().h, dtype_code(T), A.padded[1], A.padded[2], A.data.ptr, ld(A), N), "mod!")
+    return A
+end
+
+"""
+    change_modulus(A, new_N)
+
+A copy of A under modulus new_N, its entries reduced mod new_N when new_N <
+A.N and copied as they are otherwise (CuModMatrix.jl:723-739).
+"""
+function change_modulus(A::ModMatrix{T}, new_N::Integer) where {T}
+    R = ModMatrix{T}(rows(A), cols(A), new_N; zero=false)
+    check(ccall((:modp_memcpy_d2d, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Csize_t),
+                context().h, R.data.ptr, A.data.ptr, A.data.nbytes), "copy")
+    new_N < A.N && mod_elements!(R)
+    return R
+end
+
+"""
+    change_modulus_no_alloc!(A, new_N)
+
+A's storage reduced in place when new_N < A.N, returned under modulus new_N
+(CuModMatrix.jl:751-758).
+"""
+function change_modulus_no_alloc!(A::ModMatrix{T}, new_N::Integer) where {T}
+    new_N < A.N && mod_elements!(A, new_N)
+    return ModMatrix{T}(A.data, A.padded, A.size, new_N)
+end
 
 # ---- GEMM (stripe_mul.jl:13-27, 175-244; CuModMatrix.jl:767-787; mul_ops.jl:54-58)
 function find_max_stripe_ops(T, N)
@@ -187,6 +240,28 @@ function Base.:*(A::ModMatrix{T}, B::ModMatrix{T}) where {T}
     return C
 end
 
+"""
+    A^n
+
+Binary exponentiation as in CuModMatrix.jl:307-329: a square matrix only;
+A^0 is the identity, a negative power is a power of the inverse.
+"""
+function Base.:^(A::ModMatrix{T}, n::Integer) where {T}
+    rows(A) != cols(A) && throw(CuModMatrixNotSquareException("Matrix must be square for power operation"))
+    if n == 0
+        return eye(T, rows(A), A.N)
+    elseif n < 0
+        return inverse(A)^(-n)
+    elseif n == 1
+        return A
+    elseif n % 2 == 0
+        H = A^(n ÷ 2)
+        return H * H
+    else
+        return A * (A^(n - 1))
+    end
+end
+
 function mat_mul_gpu_type(A::ModMatrix{T}, B::ModMatrix{T}, mod_N::Integer=-1) where {T}
     N = mod_N > 0 ? mod_N : A.N
     C = zeros(T, rows(A), cols(B), N)
@@ -207,7 +282,7 @@ function mod_inv(p, P)
     return inv < 0 ? inv + P : inv
 end
 
-stack(buf, n) = [(Int(buf[2i-1]), Int(buf[2i])) for i in 1:n]
+swap_stack(buf, n) = [(Int(buf[2i-1]), Int(buf[2i])) for i in 1:n]
 
 function pluq(A::ModMatrix{T}; form=FORM_AUTO) where {T}
     m, n = size(A)
@@ -221,7 +296,7 @@ function pluq(A::ModMatrix{T}; form=FORM_AUTO) where {T}
                  Ptr{Int64}, Ref{Int64}, Ptr{Int64}, Ref{Int64}, Ref{Int64}),
                 context().h, form, dtype_code(T), m, n, A.data.ptr, ld(A), U.data.ptr, ld(U), L.data.ptr, ld(L), A.N,
                 rs, nr, cs, nc, rk), "pluq")
-    return U, L, stack(rs, nr[]), stack(cs, nc[]), rk[]
+    return U, L, swap_stack(rs, nr[]), swap_stack(cs, nc[]), rk[]
 end
 
 """
@@ -234,7 +309,7 @@ function pluq_gpu_kernel(A::ModMatrix; debug::Bool=false)
     return U, L, P, Q
 end
 
-function lu_ple(A::ModMatrix{T}; form=FORM_AUTO) where {T}
+function lu_ple(A::ModMatrix{T}; form=FORM_AUTO, N::Integer=A.N) where {T}
     m, n = size(A)
     U = zeros(T, m, n, A.N)
     L = zeros(T, m, m, A.N)
@@ -244,10 +319,21 @@ function lu_ple(A::ModMatrix{T}; form=FORM_AUTO) where {T}
     check(ccall((:modp_lu, LIB), Cint,
                 (Ptr{Cvoid}, Cint, Cint, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64,
                  Ptr{Int64}, Ptr{Int64}, Ref{Int64}),
-                context().h, form, dtype_code(T), m, n, A.data.ptr, ld(A), U.data.ptr, ld(U), L.data.ptr, ld(L), A.N,
+                context().h, form, dtype_code(T), m, n, A.data.ptr, ld(A), U.data.ptr, ld(U), L.data.ptr, ld(L), N,
                 perm, piv, rk), "lu")
     return U, L, perm[1:m]
 end
+
+"""
+    lu(A::ModMatrix) -> (U, L, perm)
+
+PLE with row pivoting (largest residue, first row on ties), zero columns
+skipped: A[perm, :] == L * U mod N.  The call surface of lu_gpu_type
+(test/Experiments/rref_gpu_type.jl:60-103), which takes an optional modulus
+override.
+"""
+LinearAlgebra.lu(A::ModMatrix; form=FORM_AUTO) = lu_ple(A; form=form)
+lu_gpu_type(A::ModMatrix, mod_N::Integer=-1) = lu_ple(A; N=(mod_N > 0 ? mod_N : A.N))
 
 function rref(A::ModMatrix{T}; form=FORM_AUTO) where {T}
     m, n = size(A)
@@ -260,18 +346,27 @@ function rref(A::ModMatrix{T}; form=FORM_AUTO) where {T}
     return R
 end
 
+# the call surface of rref_gpu_type (test/Experiments/rref_gpu_type.jl:8-51)
+rref_gpu_type(A::ModMatrix) = rref(A)
+
 # ---- permutations (permutations.jl:11-125) ------------------------------------
-function perm!(fn::Symbol, P::Vector{Tuple{Int,Int}}, A::ModMatrix{T}, inverse::Integer) where {T}
-    st = Int64[x for t in P for x in t]
-    check(ccall((fn, LIB), Cint, (Ptr{Cvoid}, Cint, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Int64}, Int64, Cint),
-                context().h, dtype_code(T), A.padded[1], A.padded[2], A.data.ptr, ld(A), st, length(P), inverse), String(fn))
-    return nothing
+# ccall needs its (function, library) pair to be a constant: one method per
+# C entry point, generated here
+for (jlname, cname) in ((:row_perm!, :modp_apply_row_perm), (:col_perm!, :modp_apply_col_perm))
+    @eval function $jlname(P::Vector{Tuple{Int,Int}}, A::ModMatrix{T}, inverse::Integer) where {T}
+        st = Int64[x for t in P for x in t]
+        check(ccall(($(QuoteNode(cname)), LIB), Cint,
+                    (Ptr{Cvoid}, Cint, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Int64}, Int64, Cint),
+                    context().h, dtype_code(T), A.padded[1], A.padded[2], A.data.ptr, ld(A), st, length(P), inverse),
+              $(string(cname)))
+        return nothing
+    end
 end
 
-apply_row_perm!(P, A::ModMatrix) = perm!(:modp_apply_row_perm, P, A, 0)
-apply_row_inv_perm!(P, A::ModMatrix) = perm!(:modp_apply_row_perm, P, A, 1)
-apply_col_perm!(P, A::ModMatrix) = perm!(:modp_apply_col_perm, P, A, 0)
-apply_col_inv_perm!(P, A::ModMatrix) = perm!(:modp_apply_col_perm, P, A, 1)
+apply_row_perm!(P, A::ModMatrix) = row_perm!(P, A, 0)
+apply_row_inv_perm!(P, A::ModMatrix) = row_perm!(P, A, 1)
+apply_col_perm!(P, A::ModMatrix) = col_perm!(P, A, 0)
+apply_col_inv_perm!(P, A::ModMatrix) = col_perm!(P, A, 1)
 
 # ---- inverses (triangular_inverse_no_copy.jl:197-229,450-478; CuModMatrix.jl:340-502) ----
 function triangular_inverse(A::ModMatrix{T}, uplo::Integer) where {T}

@@ -291,8 +291,9 @@ int modp_kernel_timing_reset(modp_ctx *ctx);
 int modp_event_record(modp_ctx *ctx, int slot);
 int modp_event_elapsed(modp_ctx *ctx, int slot_start, int slot_end, float *ms);
 /* Measured device peak (no counterpart in the reference; for roofline
- * reports): kind 0 = dense int8 MFMA ops/s, 1 = FP64 MFMA flop/s,
- * 2 = HBM streaming bytes/s (1 GiB read + 1 GiB written). */
+ * reports): kind 0 = dense int8 MFMA ops/s (v_mfma_i32_16x16x64_i8),
+ * 1 = FP64 MFMA flop/s, 2 = HBM streaming bytes/s (1 GiB read + 1 GiB
+ * written), 3 = dense int8 MFMA ops/s (v_mfma_i32_32x32x32_i8). */
 int modp_probe_peak(modp_ctx *ctx, int kind, double *rate);
 
 #ifdef __cplusplus

@@ -33,6 +33,29 @@ _u64 = ctypes.c_uint64
 _p64 = ctypes.POINTER(ctypes.c_int64)
 
 
+def _bind(L):
+    """Declare the C signatures on a loaded liboracle (or a test-built variant)."""
+    L.oracle_draw.restype = _u64
+    L.oracle_draw.argtypes = [_u64, _u64, _u64]
+    L.oracle_fill.restype = None
+    L.oracle_fill.argtypes = [_p64, _i64, _i64, _i64, _u64, _u64, _u64]
+    L.oracle_mod_inv.restype = _i64
+    L.oracle_mod_inv.argtypes = [_i64, _i64]
+    L.oracle_gemm.restype = None
+    L.oracle_gemm.argtypes = [_i64, _i64, _i64, _p64, _i64, _p64, _i64, _p64, _i64, _i64]
+    L.oracle_pluq.restype = ctypes.c_int
+    L.oracle_pluq.argtypes = [ctypes.c_int, _i64, _i64, _p64, _i64, _i64,
+                              _p64, _i64, _p64, _i64, _p64, _p64, _p64, _p64, _i64, _p64]
+    L.oracle_lu.restype = ctypes.c_int
+    L.oracle_lu.argtypes = [_i64, _i64, _p64, _i64, _i64, _p64, _i64, _p64, _i64,
+                            _p64, _p64, _p64]
+    L.oracle_rref.restype = ctypes.c_int
+    L.oracle_rref.argtypes = [_i64, _i64, _p64, _i64, _i64, _p64, _i64, _p64, _p64]
+    L.oracle_is_prime.restype = ctypes.c_int
+    L.oracle_is_prime.argtypes = [_u64]
+    return L
+
+
 def lib():
     """Load liboracle.so (built by ``make -C oracle``)."""
     global _LIB
@@ -40,27 +63,15 @@ def lib():
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built: run `make -C oracle`")
-        L = ctypes.CDLL(path)
-        L.oracle_draw.restype = _u64
-        L.oracle_draw.argtypes = [_u64, _u64, _u64]
-        L.oracle_fill.restype = None
-        L.oracle_fill.argtypes = [_p64, _i64, _i64, _i64, _u64, _u64, _u64]
-        L.oracle_mod_inv.restype = _i64
-        L.oracle_mod_inv.argtypes = [_i64, _i64]
-        L.oracle_gemm.restype = None
-        L.oracle_gemm.argtypes = [_i64, _i64, _i64, _p64, _i64, _p64, _i64, _p64, _i64, _i64]
-        L.oracle_pluq.restype = ctypes.c_int
-        L.oracle_pluq.argtypes = [ctypes.c_int, _i64, _i64, _p64, _i64, _i64,
-                                  _p64, _i64, _p64, _i64, _p64, _p64, _p64, _p64, _i64, _p64]
-        L.oracle_lu.restype = ctypes.c_int
-        L.oracle_lu.argtypes = [_i64, _i64, _p64, _i64, _i64, _p64, _i64, _p64, _i64,
-                                _p64, _p64, _p64]
-        L.oracle_rref.restype = ctypes.c_int
-        L.oracle_rref.argtypes = [_i64, _i64, _p64, _i64, _i64, _p64, _i64, _p64, _p64]
-        L.oracle_is_prime.restype = ctypes.c_int
-        L.oracle_is_prime.argtypes = [_u64]
-        _LIB = L
+        _LIB = _bind(ctypes.CDLL(path))
     return _LIB
+
+
+def load_variant(path: str):
+    """Load another build of modp_oracle.c (the tests build planted-mistake
+    variants with -D switches) and return its bound library, to pass as
+    ``lib_`` to pluq / lu."""
+    return _bind(ctypes.CDLL(path))
 
 
 def _ptr(a: np.ndarray):
@@ -107,7 +118,7 @@ def gemm(A, B, p: int) -> np.ndarray:
     return C
 
 
-def pluq(A, p: int, mode: int = PLUQ_OPTC) -> dict:
+def pluq(A, p: int, mode: int = PLUQ_OPTC, lib_=None) -> dict:
     """PLUQ of A mod p.  Returns dict(U, L, row_swaps, col_swaps, rank).
 
     L is m x m (for src mode the columns beyond m that the reference's
@@ -121,7 +132,7 @@ def pluq(A, p: int, mode: int = PLUQ_OPTC) -> dict:
     rs = np.zeros(2 * cap, dtype=np.int64)
     cs = np.zeros(2 * cap, dtype=np.int64)
     nr = _i64(); nc = _i64(); rank = _i64()
-    rc = lib().oracle_pluq(mode, m, n, _ptr(A), max(m, 1), p, _ptr(U), max(m, 1), _ptr(L), max(m, 1),
+    rc = (lib_ or lib()).oracle_pluq(mode, m, n, _ptr(A), max(m, 1), p, _ptr(U), max(m, 1), _ptr(L), max(m, 1),
                            _ptr(rs), ctypes.byref(nr), _ptr(cs), ctypes.byref(nc), cap, ctypes.byref(rank))
     if rc != 0:
         raise RuntimeError("oracle_pluq stack overflow")
@@ -133,7 +144,7 @@ def pluq(A, p: int, mode: int = PLUQ_OPTC) -> dict:
     )
 
 
-def lu(A, p: int) -> dict:
+def lu(A, p: int, lib_=None) -> dict:
     """PLE: A[perm-1, :] = L @ U mod p.  Returns dict(U, L, perm, pivcols, rank)."""
     A = _as_f64(A)
     m, n = A.shape
@@ -142,7 +153,7 @@ def lu(A, p: int) -> dict:
     perm = np.zeros(max(m, 1), dtype=np.int64)
     piv = np.zeros(max(min(m, n), 1), dtype=np.int64)
     rank = _i64()
-    lib().oracle_lu(m, n, _ptr(A), max(m, 1), p, _ptr(U), max(m, 1), _ptr(L), max(m, 1),
+    (lib_ or lib()).oracle_lu(m, n, _ptr(A), max(m, 1), p, _ptr(U), max(m, 1), _ptr(L), max(m, 1),
                     _ptr(perm), _ptr(piv), ctypes.byref(rank))
     r = int(rank.value)
     return dict(U=U, L=L, perm=perm[:m].copy(), pivcols=piv[:r].copy(), rank=r)
@@ -368,3 +379,190 @@ def _is_prime_small(p: int) -> bool:
             return False
         i += 1
     return True
+
+
+# ---------------------------------------------------------------------------
+# the pivot rule, independently of the C restatement
+# ---------------------------------------------------------------------------
+
+def findmax_first(values) -> Tuple[int, int]:
+    """Base.findmax semantics (value, 0-based index of the FIRST maximal
+    element), which pluq_kernels.jl:188-189 relies on.  Implemented with
+    numpy.argmax, whose documented tie rule is the same: "in case of multiple
+    occurrences of the maximum values, the indices corresponding to the first
+    occurrence are returned"."""
+    v = np.asarray(values, dtype=np.int64)
+    i = int(np.argmax(v))
+    return int(v[i]), i
+
+
+def pluq_restated(A, p: int, mode: int = PLUQ_OPTC) -> dict:
+    """A second, pure-Python restatement of pluq_gpu_kernel
+    (pluq_kernels.jl:46-157), written from the reference in Python integers
+    with the pivot search delegated to findmax_first.  mode PLUQ_SRC follows
+    the reference loop as written (zero column: swap with the last column,
+    col += 1, L indexed by col); PLUQ_OPTC applies the corrected zero-column
+    handling (swap with a shrinking last column and retry; all-zero block:
+    the tail of swaps (c, n)) and indexes L by the pivot count.  Small inputs
+    only: it exists to pin the C restatement's pivot choices."""
+    M = [[int(x) % p for x in row] for row in np.asarray(A, dtype=np.int64).tolist()]
+    m = len(M)
+    n = len(M[0]) if m else 0
+    lcols = max(m, n)
+    Lw = [[0] * lcols for _ in range(m)]
+    SR, SC = [], []
+    row = col = 0
+    last = n - 1
+    # swap_rows(d_L, ...) is launched over 32*cld(rows,32) columns (:254,:282)
+    lswap = min(lcols, 32 * ((m + 31) // 32))
+    while row < m and col < n:
+        v, i = findmax_first([M[r][col] for r in range(row, m)])
+        if v == 0:
+            if mode == PLUQ_SRC:
+                # find_pivot :191-198: swap with column `cols`, push, col += 1
+                for r in range(m):
+                    M[r][col], M[r][n - 1] = M[r][n - 1], M[r][col]
+                SC.append((col + 1, n))
+                col += 1
+                continue
+            if all(M[r][c] == 0 for r in range(row, m) for c in range(col, n)):
+                for c in range(col, n):
+                    for r in range(m):
+                        M[r][c], M[r][n - 1] = M[r][n - 1], M[r][c]
+                    SC.append((c + 1, n))
+                break
+            for r in range(m):
+                M[r][col], M[r][last] = M[r][last], M[r][col]
+            SC.append((col + 1, last + 1))
+            last -= 1
+            continue
+        pr = row + i
+        inv = pow(v, -1, p)
+        # swap_rows_and_mod(d_A, prow, row, ...) :307-320
+        old = M[pr]
+        M[pr] = M[row]
+        M[row] = [(x * inv) % p for x in old]
+        if pr != row:
+            for c in range(lswap):
+                Lw[row][c], Lw[pr][c] = Lw[pr][c], Lw[row][c]
+            SR.append((row + 1, pr + 1))
+        lc = col if mode == PLUQ_SRC else row
+        # move_and_zero_out :340-351
+        Lw[row][lc] = v
+        mult = [0] * m
+        for r in range(row + 1, m):
+            Lw[r][lc] = M[r][col]
+            mult[r] = M[r][col]
+            M[r][col] = 0
+        # update_sub_matrix_kernel :411-440
+        for r in range(row + 1, m):
+            f = p - mult[r]
+            for c in range(col + 1, n):
+                M[r][c] = (M[r][c] + f * M[row][c]) % p
+        row += 1
+        col += 1
+    U = np.array(M, dtype=np.int64).reshape(m, n)
+    Lfull = np.array(Lw, dtype=np.int64).reshape(m, lcols)
+    return dict(U=U, L=Lfull[:, :m].copy(), L_extra=Lfull[:, m:].copy(), row_swaps=SR, col_swaps=SC, rank=row)
+
+
+def lu_restated(A, p: int) -> dict:
+    """Pure-Python PLE with the same pivot rule (findmax_first): a zero column
+    is skipped without a column swap; returns dict(U, L, perm, pivcols, rank)
+    like the C lu (intent of test/Experiments/rref_gpu_type.jl:60-103)."""
+    M = [[int(x) % p for x in row] for row in np.asarray(A, dtype=np.int64).tolist()]
+    m = len(M)
+    n = len(M[0]) if m else 0
+    Lw = [[0] * m for _ in range(m)]
+    perm = list(range(1, m + 1))
+    piv = []
+    row = col = 0
+    while row < m and col < n:
+        v, i = findmax_first([M[r][col] for r in range(row, m)])
+        if v == 0:
+            col += 1
+            continue
+        pr = row + i
+        inv = pow(v, -1, p)
+        old = M[pr]
+        M[pr] = M[row]
+        M[row] = [(x * inv) % p for x in old]
+        if pr != row:
+            Lw[row], Lw[pr] = Lw[pr], Lw[row]
+            perm[row], perm[pr] = perm[pr], perm[row]
+        Lw[row][row] = v
+        for r in range(row + 1, m):
+            f = M[r][col]
+            Lw[r][row] = f
+            M[r][col] = 0
+            for c in range(col + 1, n):
+                M[r][c] = (M[r][c] - f * M[row][c]) % p
+        piv.append(col + 1)
+        row += 1
+        col += 1
+    return dict(U=np.array(M, dtype=np.int64).reshape(m, n), L=np.array(Lw, dtype=np.int64).reshape(m, m),
+                perm=np.array(perm, dtype=np.int64), pivcols=np.array(piv, dtype=np.int64), rank=row)
+
+
+def pivot_targets_from_stack(row_swaps, rank: int) -> List[int]:
+    """0-based source row of each pivot from a PLUQ row stack: the swap pushed
+    at pivot k is (k+1, at+1); a pivot without an entry did not move."""
+    at = list(range(rank))
+    for a, b in row_swaps:
+        at[a - 1] = b - 1
+    return at
+
+
+def pivot_targets_from_perm(perm, rank: int) -> List[int]:
+    """0-based source row of each pivot of a PLE from its final row
+    permutation.  Pivot k swaps positions k and at_k >= k, and later pivots
+    never touch position k again, so the decomposition is unique: replay it
+    forwards."""
+    perm = [int(x) for x in perm]
+    cur = list(range(1, len(perm) + 1))
+    where = {v: i for i, v in enumerate(cur)}
+    at = []
+    for k in range(rank):
+        j = where[perm[k]]
+        if j < k:
+            raise ValueError("perm is not a forward transposition sequence")
+        at.append(j)
+        vk, vj = cur[k], cur[j]
+        cur[k], cur[j] = vj, vk
+        where[vj], where[vk] = k, j
+    return at
+
+
+def check_pivot_rule(L, rank: int, at: List[int]) -> List[str]:
+    """The pivot rule of pluq_kernels.jl:188-189 (largest residue, FIRST
+    position among equal residues), checked on a factorisation's L.
+
+    Column k of L holds the pivot L[k,k] and, below it, the residues of the
+    pivot column at step k -- in the FINAL row order.  Undoing the row swaps
+    of pivots k+1.. (from the last pivot down) restores the order of step k.
+    In that order, with the step's own swap (k, at_k) undone as well, the
+    pivot is the first maximum of rows k..m-1: every row in k..at_k-1 is
+    strictly smaller, every row after at_k is at most equal."""
+    L = np.array(L, dtype=np.int64, copy=True)
+    m = L.shape[0]
+    errs = []
+    for k in range(rank - 1, -1, -1):
+        col = L[:, k].copy()
+        a = at[k]
+        v = col[k]
+        # order of step k before its own swap: rows k and a exchanged back
+        col[k], col[a] = col[a], col[k]
+        seg = col[k:]
+        if v <= 0 or seg[a - k] != v:
+            errs.append(f"pivot {k}: value bookkeeping")
+            break
+        if np.any(seg > v):
+            errs.append(f"pivot {k}: a larger residue was below the pivot")
+            break
+        if np.any(seg[:a - k] == v):
+            errs.append(f"pivot {k}: an earlier row had an equal residue (tie not broken to the first)")
+            break
+        # undo this pivot's swap on the rows of L for the next (earlier) step
+        if a != k:
+            L[[k, a], :] = L[[a, k], :]
+    return errs

@@ -197,13 +197,20 @@ static work_t work_load(int64_t m, int64_t n, const int64_t *A, int64_t lda, int
     return w;
 }
 
-/* argmax of W[row:m, col]: largest value, lowest row on ties */
+/* argmax of W[row:m, col]: largest value, lowest row on ties.
+ * ORACLE_PLANT_TIE_LAST builds a deliberately wrong variant (highest row on
+ * ties) that only the tests compile, to show that the tie-break checks catch
+ * it; the shipped liboracle.so never defines it. */
 static void col_argmax(const work_t *w, int64_t row, int64_t col, uint64_t *v, int64_t *at)
 {
     uint64_t best = 0; int64_t bi = row;
     for (int64_t r = row; r < w->m; ++r) {
         uint64_t x = w->W[r * w->n + col];
+#ifdef ORACLE_PLANT_TIE_LAST
+        if (x > 0 && x >= best) { best = x; bi = r; }
+#else
         if (x > best) { best = x; bi = r; }
+#endif
     }
     *v = best; *at = bi;
 }

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from gpufinitefieldmatrices import workloads as W
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -104,29 +105,6 @@ def test_batched_gemm_edge_shapes(gffm):
         assert np.array_equal(g.Array(C).astype(np.int64), O.gemm(Ah, Bh, p))
 
 
-def make_elim_instances(count, seed, maxdim=128):
-    rng = np.random.default_rng(seed)
-    primes = [2, 3, 7, 251, 65521]
-    out = []
-    for i in range(count):
-        m, n = (int(x) for x in rng.integers(8, maxdim + 1, size=2))
-        p = primes[int(rng.integers(0, 5))]
-        A = O.draw_matrix(m, n, p, seed=seed * 100000 + i)
-        if i % 4 == 0:
-            # rank deficient: zero a random row subset, then left-multiply by
-            # an invertible matrix
-            k = int(rng.integers(1, m))
-            rowsz = rng.choice(m, size=k, replace=False)
-            A[rowsz, :] = 0
-            for t in range(100):
-                S = O.draw_matrix(m, m, p, seed=seed * 100000 + i + 7919 * (t + 1))
-                if O.rref(S, p)["rank"] == m:
-                    break
-            A = O.gemm(S, A, p)
-        out.append((A, p))
-    return out
-
-
 class ElimBatch:
     def __init__(self, g, inst, op):
         self.g, self.inst, self.op = g, inst, op
@@ -164,7 +142,7 @@ class ElimBatch:
 @pytest.mark.parametrize("op", ["pluq", "lu", "rref"])
 def test_batched_elim_vs_oracle(gffm, op):
     g = gffm
-    inst = make_elim_instances(300, seed=4)
+    inst = W.elim_instances(300, seed=4)
     code = {"pluq": 0, "lu": 1, "rref": 2}[op]
     b = ElimBatch(g, inst, code)
     b.run()
@@ -190,3 +168,95 @@ def test_batched_elim_vs_oracle(gffm, op):
             assert rank == ref["rank"]
             assert r["row"][:rank].tolist() == ref["pivcols"].tolist()
             assert np.array_equal(r["U"], ref["R"])
+
+
+# ---------------------------------------------------------------- full batches
+# The whole batched workloads that the benchmark times, every instance
+# compared with the oracle.
+
+def test_full_gemm_batch_vs_oracle(gffm):
+    # 10,000 products, m, n, k ~ U[16, 256], each under one of the 64 largest
+    # primes below 2^23 (seed 3); the graph is replayed and every replay's
+    # outputs are compared
+    g = gffm
+    ctx = g.context()
+    wb = W.GemmBatch(10000, seed=3)
+    print(f"full gemm batch: {wb.count} products, {wb.elems * 8 / 1e9:.2f} GB arena", flush=True)
+    buf = ctx.malloc(8 * wb.elems)
+    buf.upload(wb.arena)
+    jobs = W.gemm_job_table(wb, buf.ptr)
+    import ctypes
+    h = ctypes.c_void_p()
+    ctx.check(ctx.lib.modp_gemm_batched_plan(ctx.h, 0, jobs, wb.count, ctypes.byref(h)))
+    try:
+        refs = []
+        for i in range(wb.count):
+            A, B, p = wb.operands(i)
+            refs.append(O.gemm(A, B, p))
+        out = np.empty(wb.elems, dtype=np.float64)
+        for rep in range(2):
+            ctx.check(ctx.lib.modp_gemm_batched_run(ctx.h, h))
+            ctx.synchronize()
+            buf.download(out)
+            bad = []
+            for i in range(wb.count):
+                m, n, _ = (int(x) for x in wb.shapes[i])
+                c = wb.c_off(i)
+                C = out[c:c + m * n].reshape(n, m).T.astype(np.int64)
+                if not np.array_equal(C, refs[i]):
+                    bad.append(i)
+            assert bad == [], f"replay {rep}: {len(bad)} products differ, first {bad[:5]}"
+            # eager work between replays: clear every C
+            buf.upload(wb.arena)
+    finally:
+        ctx.check(ctx.lib.modp_gemm_batched_destroy(ctx.h, h))
+
+
+@pytest.fixture(scope="module")
+def full_elim_batch():
+    return W.elim_instances(4096, seed=4)
+
+
+def _elim_mismatches(op, res, inst):
+    bad = []
+    for i, ((A, p), r) in enumerate(zip(inst, res)):
+        m, n = A.shape
+        if op == "pluq":
+            ref = O.pluq(A, p, O.PLUQ_OPTC)
+            nr, nc, rank = (int(x) for x in r["counts"])
+            ok = (rank == ref["rank"]
+                  and [tuple(x) for x in r["row"][:2 * nr].reshape(-1, 2).tolist()] == ref["row_swaps"]
+                  and [tuple(x) for x in r["col"][:2 * nc].reshape(-1, 2).tolist()] == ref["col_swaps"]
+                  and np.array_equal(r["U"], ref["U"]) and np.array_equal(r["L"], ref["L"]))
+        elif op == "lu":
+            ref = O.lu(A, p)
+            rank = int(r["counts"][2])
+            ok = (rank == ref["rank"] and r["row"][:m].tolist() == ref["perm"].tolist()
+                  and r["col"][:rank].tolist() == ref["pivcols"].tolist()
+                  and np.array_equal(r["U"], ref["U"]) and np.array_equal(r["L"], ref["L"]))
+        else:
+            ref = O.rref(A, p)
+            rank = int(r["counts"][2])
+            ok = (rank == ref["rank"] and r["row"][:rank].tolist() == ref["pivcols"].tolist()
+                  and np.array_equal(r["U"], ref["R"]))
+        if not ok:
+            bad.append(i)
+    return bad
+
+
+@pytest.mark.parametrize("op", ["pluq", "lu", "rref"])
+def test_full_elim_batch_vs_oracle(gffm, full_elim_batch, op):
+    # 4096 eliminations, rows and columns ~ U[8, 128], p in {2, 3, 7, 251,
+    # 65521}, every fourth rank-deficient (seed 4): every instance's outputs
+    # equal the oracle's
+    g = gffm
+    ctx = g.context()
+    print(f"full elimination batch: {op}", flush=True)
+    arena = W.ElimArena(ctx, full_elim_batch)
+    code = {"pluq": 0, "lu": 1, "rref": 2}[op]
+    for rep in range(2):
+        arena.clear_outputs()
+        g.elim_batched(code, list(arena.jobs), ctx=ctx)
+        ctx.synchronize()
+        bad = _elim_mismatches(op, arena.download(), full_elim_batch)
+        assert bad == [], f"run {rep}: {len(bad)} instances differ, first {bad[:5]}"

@@ -204,37 +204,69 @@ def test_not_prime_modulus_is_an_error(gffm):
             f(dev(g, Ah, 4))
 
 
-@pytest.mark.parametrize("kind", ["full", "rank3000"])
-def test_config3_4096_invariants(gffm, kind):
-    # full size: the factorisation identity by Freivalds, structure, rank, and
-    # plain vs limb trailing updates bit-identical
-    g = gffm
+_C3 = {}
+
+
+def _config3_input(g, kind):
+    """The 4096^2 elimination inputs mod 65521: uniform residues (seed 2,
+    full rank) or the product of a 4096x3000 and a 3000x4096 uniform factor
+    (seed 2, rank 3000), with the product checked against the oracle."""
     p, n = 65521, 4096
-    print(f"config3 {kind}", flush=True)
-    if kind == "full":
-        A = g.rand(np.float64, n, n, p, seed=2)
-    else:
-        X = g.rand(np.float64, n, 3000, p, seed=2)
-        Y = g.rand(np.float64, 3000, n, p, seed=2, offset=n * 3000)
-        A = X * Y
-    U1, L1, P1, Q1, r1 = g.pluq(A, form=1)
-    U2, L2, P2, Q2, r2 = g.pluq(A, form=2)
-    assert r1 == r2 == (n if kind == "full" else 3000)
-    assert P1 == P2 and Q1 == Q2
-    u1, u2 = g.Array(U1), g.Array(U2)
-    assert np.array_equal(u1, u2)
-    l1 = g.Array(L1)
-    assert np.array_equal(l1, g.Array(L2))
-    Ah = g.Array(A).astype(np.int64)
-    errs = O.check_pluq(Ah, p, u1.astype(np.int64), l1.astype(np.int64), P1, Q1, r1, exact=False)
-    assert errs == []
-    if kind == "rank3000":
-        R, piv, rank = g.rref_full(A)
-        assert rank == 3000
-        Rh = g.Array(R).astype(np.int64)
-        ok, piv2 = O.is_rref(Rh[:3000 + 1, :], p)
-        assert ok and piv2 == piv
-        # rows of A lie in the row space of R: A == A[:, piv] * R
-        Ap = Ah[:, [c - 1 for c in piv]]
-        assert O.freivalds_equal(lambda x: O._mod_matvec(Ah, x, p),
-                                 lambda x: O._mod_matvec(Ap, O._mod_matvec(Rh[:3000], x, p), p), n, p)
+    if kind not in _C3:
+        if kind == "full":
+            A = g.rand(np.float64, n, n, p, seed=2)
+        else:
+            X = g.rand(np.float64, n, 3000, p, seed=2)
+            Y = g.rand(np.float64, 3000, n, p, seed=2, offset=n * 3000)
+            A = X * Y
+            assert np.array_equal(g.Array(A).astype(np.int64),
+                                  O.gemm(g.Array(X).astype(np.int64), g.Array(Y).astype(np.int64), p))
+        _C3[kind] = (A, g.Array(A).astype(np.int64))
+    return _C3[kind]
+
+
+@pytest.mark.parametrize("kind", ["full", "rank3000"])
+@pytest.mark.parametrize("form", [1, 2])
+def test_config3_4096_pluq_vs_oracle(gffm, kind, form):
+    # full size: U, L, both swap stacks and the rank equal the oracle's, and
+    # the pivots follow the reference's rule (largest residue, first row)
+    g = gffm
+    p = 65521
+    print(f"config3 pluq {kind} form {form}", flush=True)
+    A, Ah = _config3_input(g, kind)
+    U, L, P, Q, rank = g.pluq(A, form=form)
+    ref = O.pluq(Ah, p, O.PLUQ_OPTC)
+    assert rank == ref["rank"] == (4096 if kind == "full" else 3000)
+    assert P == ref["row_swaps"] and Q == ref["col_swaps"]
+    Lh = g.Array(L).astype(np.int64)
+    assert np.array_equal(g.Array(U).astype(np.int64), ref["U"])
+    assert np.array_equal(Lh, ref["L"])
+    assert O.check_pivot_rule(Lh, rank, O.pivot_targets_from_stack(P, rank)) == []
+
+
+@pytest.mark.parametrize("kind", ["full", "rank3000"])
+def test_config3_4096_lu_vs_oracle(gffm, kind):
+    g = gffm
+    p = 65521
+    print(f"config3 lu {kind}", flush=True)
+    A, Ah = _config3_input(g, kind)
+    U, L, perm, piv, rank = g.lu_full(A)
+    ref = O.lu(Ah, p)
+    assert rank == ref["rank"]
+    assert list(perm) == ref["perm"].tolist() and list(piv) == ref["pivcols"].tolist()
+    Lh = g.Array(L).astype(np.int64)
+    assert np.array_equal(g.Array(U).astype(np.int64), ref["U"])
+    assert np.array_equal(Lh, ref["L"])
+    assert O.check_pivot_rule(Lh, rank, O.pivot_targets_from_perm(perm, rank)) == []
+
+
+@pytest.mark.parametrize("kind", ["full", "rank3000"])
+def test_config3_4096_rref_vs_oracle(gffm, kind):
+    g = gffm
+    p = 65521
+    print(f"config3 rref {kind}", flush=True)
+    A, Ah = _config3_input(g, kind)
+    R, piv, rank = g.rref_full(A)
+    ref = O.rref(Ah, p)
+    assert rank == ref["rank"] and list(piv) == ref["pivcols"].tolist()
+    assert np.array_equal(g.Array(R).astype(np.int64), ref["R"])

@@ -191,28 +191,25 @@ def test_device_rand_matches_oracle_generator(gffm):
 
 
 @pytest.mark.parametrize("p", [2 ** 26 - 5, 127])
-def test_config2_8192_freivalds_and_forms_agree(gffm, p):
-    # full size: limb and plain forms bit-identical, and C x == A (B x) mod p
+def test_config2_8192_vs_oracle(gffm, p):
+    # full size: the 8192^3 product of the seeded residues (seed 1) in both
+    # forms equals the oracle's product entry for entry (max |delta| = 0)
     g = gffm
     n = 8192
     print(f"config2 n={n} p={p}", flush=True)
     A = g.rand(np.float64, n, n, p, seed=1)
     B = g.rand(np.float64, n, n, p, seed=1, offset=n * n)
-    C1 = g.zeros(np.float64, n, n, p)
-    C2 = g.zeros(np.float64, n, n, p)
-    g.mul_(C1, A, B, form=1)
-    g.mul_(C2, A, B, form=2)
-    c1, c2 = g.Array(C1), g.Array(C2)
-    assert int(np.abs(c1 - c2).max()) == 0
     Ah = g.Array(A).astype(np.int64)
     Bh = g.Array(B).astype(np.int64)
-    Ch = c2.astype(np.int64)
-    ok = O.freivalds_equal(lambda x: O._mod_matvec(Ch, x, p),
-                           lambda x: O._mod_matvec(Ah, O._mod_matvec(Bh, x, p), p), n, p, trials=2)
-    assert ok
-    # a block against the oracle exactly
-    blk = O.gemm(Ah[:256], Bh[:, :256], p)
-    assert np.array_equal(Ch[:256, :256], blk)
+    ref = O.gemm(Ah, Bh, p)
+    del Ah, Bh
+    for form in (1, 2):
+        C = g.zeros(np.float64, n, n, p)
+        g.mul_(C, A, B, form=form)
+        c = g.Array(C).astype(np.int64)
+        del C
+        assert int(np.abs(c - ref).max()) == 0, f"form {form}"
+        del c
 
 
 @pytest.mark.parametrize("p", [2, 127, 65521, 8388593, 2 ** 26 - 5])

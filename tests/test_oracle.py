@@ -329,27 +329,11 @@ def test_is_rref_catches_planted_mistake():
     assert not O.is_rref(R2, 65521)[0]
 
 
-def _c5_instances(count=4096, seed=4):
-    """The batched-elimination workload (same construction and draw order as
-    bench.py): rows, cols ~ U[8,128], p from {2,3,7,251,65521}, every fourth
-    instance rank-deficient (a random row subset zeroed, then left-multiplied
-    by a random matrix redrawn until invertible mod p)."""
-    rng = np.random.default_rng(seed)
-    primes = [2, 3, 7, 251, 65521]
-    out = []
-    for i in range(count):
-        m, n = (int(x) for x in rng.integers(8, 129, size=2))
-        p = primes[int(rng.integers(0, 5))]
-        A = rng.integers(0, p, size=(m, n))
-        if i % 4 == 0:
-            A[rng.choice(m, size=int(rng.integers(1, m)), replace=False), :] = 0
-            while True:
-                S = rng.integers(0, p, size=(m, m))
-                if O.pluq(S, p)["rank"] == m:
-                    break
-            A = O.gemm(S, A, p)
-        out.append((np.asarray(A, dtype=np.int64), p))
-    return out
+def _c5_instances():
+    # the batched-elimination workload exactly as the benchmark and the GPU
+    # tests build it
+    from gpufinitefieldmatrices import workloads as W
+    return W.elim_instances(4096, seed=4)
 
 
 def test_c5_src_and_optc_differences():
@@ -402,3 +386,110 @@ def test_oracle_inverse_reference_fixtures():
     assert (O.inverse(np.eye(50, dtype=np.int64), 2) == np.eye(50, dtype=np.int64)).all()
     # the upper inverse is upper triangular, the lower one lower triangular
     assert O.inverse(U, 7)[1, 0] == 0 and O.inverse(L, 7)[0, 1] == 0
+
+
+# ---------------------------------------------------------------- pivot rule
+# The reference picks the pivot with Base.findmax over the column
+# (pluq_kernels.jl:188-189): the largest residue, the FIRST row among equal
+# residues.  Two independent pins of the C restatement's choice: a pure-Python
+# restatement whose search is numpy.argmax (documented first-occurrence rule),
+# compared on every tie-heavy input, and an invariant checker that reads the
+# rule off the factorisation itself.
+
+def _same_pluq(a, b):
+    return (a["rank"] == b["rank"] and a["row_swaps"] == b["row_swaps"] and a["col_swaps"] == b["col_swaps"]
+            and np.array_equal(a["U"], b["U"]) and np.array_equal(a["L"], b["L"])
+            and np.array_equal(a["L_extra"], b["L_extra"]))
+
+
+def _same_lu(a, b):
+    return (a["rank"] == b["rank"] and np.array_equal(a["perm"], b["perm"])
+            and np.array_equal(a["pivcols"], b["pivcols"])
+            and np.array_equal(a["U"], b["U"]) and np.array_equal(a["L"], b["L"]))
+
+
+def _pivot_rule_errors(A, p, lib_=None):
+    errs = []
+    for mode in (O.PLUQ_OPTC, O.PLUQ_SRC):
+        r = O.pluq(A, p, mode, lib_=lib_)
+        if not _same_pluq(r, O.pluq_restated(A, p, mode)):
+            errs.append(f"pluq mode {mode} differs from the restatement")
+        if mode == O.PLUQ_OPTC:
+            e = O.check_pivot_rule(r["L"], r["rank"], O.pivot_targets_from_stack(r["row_swaps"], r["rank"]))
+            errs += [f"pluq: {x}" for x in e]
+    r = O.lu(A, p, lib_=lib_)
+    if not _same_lu(r, O.lu_restated(A, p)):
+        errs.append("lu differs from the restatement")
+    errs += [f"lu: {x}" for x in O.check_pivot_rule(r["L"], r["rank"], O.pivot_targets_from_perm(r["perm"], r["rank"]))]
+    return errs
+
+
+@pytest.mark.parametrize("p", [2, 3])
+def test_pivot_rule_exhaustive_3x3(p):
+    # every 3x3 matrix over GF(2) and GF(3): nearly every pivot is a tie
+    for vals in itertools.product(range(p), repeat=9):
+        A = np.array(vals, dtype=np.int64).reshape(3, 3)
+        errs = _pivot_rule_errors(A, p)
+        assert errs == [], (A.tolist(), errs)
+
+
+@pytest.mark.parametrize("shape", [(2, 4), (4, 2), (3, 4)])
+def test_pivot_rule_exhaustive_gf2_rectangular(shape):
+    m, n = shape
+    for vals in itertools.product(range(2), repeat=m * n):
+        A = np.array(vals, dtype=np.int64).reshape(m, n)
+        assert _pivot_rule_errors(A, 2) == [], A.tolist()
+
+
+@pytest.mark.parametrize("p,shape", [(3, (8, 8)), (7, (8, 8)), (3, (6, 9)), (7, (9, 6)), (2, (10, 10))])
+def test_pivot_rule_random_tie_heavy(p, shape):
+    rng = np.random.default_rng(p * 100 + shape[0])
+    for t in range(300):
+        A = rng.integers(0, p, size=shape)
+        if t % 3 == 0:
+            A[:, rng.integers(0, shape[1])] = 0          # a zero column
+        if t % 5 == 0:
+            A[rng.integers(0, shape[0]), :] = A[rng.integers(0, shape[0]), :]  # a repeated row
+        assert _pivot_rule_errors(A, p) == [], (t, A.tolist())
+
+
+def test_pivot_rule_check_medium_sizes():
+    # the invariant checker on the C oracle at sizes the Python restatement
+    # cannot reach; it is what the GPU tests apply at 4096^2
+    for p, m, n in ((3, 200, 150), (65521, 300, 300), (2, 128, 200)):
+        A = O.draw_matrix(m, n, p, seed=p + m)
+        r = O.pluq(A, p)
+        assert O.check_pivot_rule(r["L"], r["rank"], O.pivot_targets_from_stack(r["row_swaps"], r["rank"])) == []
+        l = O.lu(A, p)
+        assert O.check_pivot_rule(l["L"], l["rank"], O.pivot_targets_from_perm(l["perm"], l["rank"])) == []
+
+
+@pytest.fixture(scope="module")
+def planted_tie_last(tmp_path_factory):
+    # the oracle rebuilt with the tie broken to the LAST row (a `>=` in its
+    # column argmax); the checks above must reject it
+    import subprocess
+    src = os.path.join(os.path.dirname(HERE), "oracle", "modp_oracle.c")
+    out = str(tmp_path_factory.mktemp("planted") / "liboracle_tie_last.so")
+    subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-std=gnu11", "-shared", "-DORACLE_PLANT_TIE_LAST",
+                    "-o", out, src], check=True)
+    return O.load_variant(out)
+
+
+def test_planted_tie_break_is_caught(planted_tie_last):
+    A = np.array([[1, 0, 1], [1, 1, 0], [0, 1, 1]], dtype=np.int64)
+    errs = _pivot_rule_errors(A, 2, lib_=planted_tie_last)
+    assert any("differs" in e for e in errs) and any("tie" in e for e in errs), errs
+    # and on the medium-size check, where only the invariant is available
+    B = O.draw_matrix(200, 150, 3, seed=5)
+    r = O.pluq(B, 3, lib_=planted_tie_last)
+    assert O.check_pivot_rule(r["L"], r["rank"], O.pivot_targets_from_stack(r["row_swaps"], r["rank"])) != []
+    l = O.lu(B, 3, lib_=planted_tie_last)
+    assert O.check_pivot_rule(l["L"], l["rank"], O.pivot_targets_from_perm(l["perm"], l["rank"])) != []
+
+
+def test_check_pivot_rule_catches_a_smaller_pivot():
+    # a valid factorisation with a non-maximal pivot: P = I on [[1,0],[2,1]]
+    # mod 5 (pivot 1 although 2 is larger)
+    L = np.array([[1, 0], [2, 1]], dtype=np.int64)
+    assert O.check_pivot_rule(L, 2, [0, 1]) != []
