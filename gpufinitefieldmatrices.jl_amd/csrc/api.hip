@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 #include "batched.h"
@@ -93,7 +94,7 @@ static hipEvent_t take_event(modp_ctx *ctx)
 
 TimedLaunch::TimedLaunch(modp_ctx *c, const char *name) : ctx(c), t(nullptr)
 {
-    if (!ctx->timing) return;
+    if (!ctx->timing || ctx->in_noalloc) return;
     ctx->timings.push_back(KernelTiming{name, take_event(ctx), take_event(ctx)});
     t = &ctx->timings.back();
     (void)hipEventRecord(t->a, ctx->stream);
@@ -173,6 +174,40 @@ int modp_device_count(int *count)
     return MODP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// the allocation-free product's scratch
+// ---------------------------------------------------------------------------
+
+// Digit planes of modp_gemm_noalloc.  Each device holds its own copy, at its
+// own address, for the life of the process.  Host code never passes `scratch`
+// itself to a kernel (on the host that name is only a stand-in): it uses the
+// address hipGetSymbolAddress returns for the device, taken once per device
+// when the first context on it is created.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) int8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+int8_t *g_scratch_ptr[MAX_DEVICES] = {nullptr};
+hipError_t g_scratch_err[MAX_DEVICES] = {hipSuccess};
+bool g_scratch_done[MAX_DEVICES] = {false};
+
+// with `device` current: the device address of its scratch (once per device)
+int8_t *scratch_address(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void *ptr = nullptr;
+        g_scratch_err[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        g_scratch_ptr[device] = g_scratch_err[device] == hipSuccess ? (int8_t *)ptr : nullptr;
+        g_scratch_done[device] = true;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
 int modp_ctx_create(int device, modp_ctx **out)
 {
     if (!out) return MODP_ERR_ARG;
@@ -187,6 +222,8 @@ int modp_ctx_create(int device, modp_ctx **out)
     modp_ctx *c = new modp_ctx();
     c->device = device;
     c->num_cus = prop.multiProcessorCount;
+    // a failure here leaves only modp_gemm_noalloc unavailable on this device
+    c->noalloc_scratch = scratch_address(device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return MODP_ERR_DEVICE;
@@ -353,6 +390,28 @@ int modp_gemm(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, int64_t 
     if (form == MODP_FORM_AUTO) return gemm_auto(ctx, g);
     if (form == MODP_FORM_LIMB_I8) return gemm_limb_i8(ctx, g);
     return gemm_plain_f64(ctx, g);
+}
+
+int modp_gemm_noalloc(modp_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda,
+                      const void *B, int64_t ldb, void *C, int64_t ldc, int64_t p)
+{
+    CTX_CHECK(ctx);
+    if (!ctx->noalloc_scratch)
+        return set_error(ctx, MODP_ERR_DEVICE, "the no-allocation scratch has no device address on device %d",
+                         ctx->device);
+    GemmArgs g;
+    MODP_TRY(gemm_prepare(ctx, dtype, m, n, k, A, lda, B, ldb, C, ldc, p, g));
+    if (m == 0 || n == 0) return MODP_OK;
+    if (k == 0) return gemm_zero(ctx, g);
+    const size_t need = gemm_limb_scratch_bytes(ctx, g);
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return set_error(ctx, MODP_ERR_ARG,
+                         "%lld x %lld x %lld needs %zu bytes of digit planes, above the %zu-byte no-allocation scratch",
+                         (long long)m, (long long)n, (long long)k, need, NOALLOC_SCRATCH_BYTES);
+    ctx->in_noalloc = true;
+    const int st = gemm_limb_i8_in(ctx, g, LimbScratch{ctx->noalloc_scratch, NOALLOC_SCRATCH_BYTES});
+    ctx->in_noalloc = false;
+    return st;
 }
 
 int modp_gemm_ex(modp_ctx *ctx, const modp_gemm_config *cfg, int dtype, int64_t m, int64_t n, int64_t k,

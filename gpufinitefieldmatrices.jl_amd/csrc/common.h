@@ -211,6 +211,11 @@ struct modp_ctx {
     hipStream_t side[NSIDE] = {nullptr};
     hipEvent_t fork_ev = nullptr;
     hipEvent_t join_ev[NSIDE] = {nullptr};
+    // device address of the library's static scratch array on this
+    // context's device (modp_gemm_noalloc), and a flag set while such a call
+    // runs (no timing events are taken then)
+    int8_t *noalloc_scratch = nullptr;
+    bool in_noalloc = false;
 };
 
 // workspace slots
@@ -288,6 +293,14 @@ struct GemmArgs {
 // heuristic configurations (gemm.hip)
 int gemm_plain_f64(modp_ctx *ctx, const GemmArgs &g);
 int gemm_limb_i8(modp_ctx *ctx, const GemmArgs &g);
+// the limb form with its digit planes in a caller's workspace (no allocation,
+// one K slice); gemm_limb_scratch_bytes is the workspace that product needs
+struct LimbScratch {
+    int8_t *base;
+    size_t bytes;
+};
+size_t gemm_limb_scratch_bytes(const modp_ctx *ctx, const GemmArgs &g);
+int gemm_limb_i8_in(modp_ctx *ctx, const GemmArgs &g, const LimbScratch &ws);
 int limb_count(int64_t p);
 int choose_form(int64_t m, int64_t n, int64_t k, int64_t p);
 GemmConfig gemm_default_config(const modp_ctx *ctx, const GemmArgs &g, int form);

@@ -271,7 +271,9 @@ int gemm_plain_cfg(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg)
     return launch_slice_sum(ctx, g, g.epi, P, sl.S);
 }
 
-int gemm_limb_cfg(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg)
+// ext: caller-provided workspace for the digit planes (no allocation); the
+// planes of each K chunk must fit in ext->bytes and the launch has one K slice
+int gemm_limb_cfg(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg, const LimbScratch *ext = nullptr)
 {
     if (g.m <= 0 || g.n <= 0) return MODP_OK;
     const int L = limb_count(g.mp.p);
@@ -304,10 +306,20 @@ int gemm_limb_cfg(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg)
         if (kc > kmax) kc = kmax;
         const int64_t Kp = round_up(kc > 0 ? kc : 1, LBK);
         int st = MODP_OK;
-        int8_t *Al = (int8_t *)ws_get(ctx, WS_ALIMB, (size_t)L * Mp * Kp, &st);
-        if (!Al) return st;
-        int8_t *Bl = (int8_t *)ws_get(ctx, WS_BLIMB, (size_t)L * Np * Kp, &st);
-        if (!Bl) return st;
+        int8_t *Al, *Bl;
+        if (ext) {
+            const size_t need = (size_t)L * (size_t)(Mp + Np) * (size_t)Kp;
+            if (need > ext->bytes || cfg.ksplit != 1)
+                return set_error(ctx, MODP_ERR_ARG, "digit planes of %zu bytes exceed the %zu-byte workspace", need,
+                                 ext->bytes);
+            Al = ext->base;
+            Bl = ext->base + (size_t)L * Mp * Kp;
+        } else {
+            Al = (int8_t *)ws_get(ctx, WS_ALIMB, (size_t)L * Mp * Kp, &st);
+            if (!Al) return st;
+            Bl = (int8_t *)ws_get(ctx, WS_BLIMB, (size_t)L * Np * Kp, &st);
+            if (!Bl) return st;
+        }
         if (g.ka == EK_F64) MODP_TRY(split_a_t<double>(ctx, g, g.A, kdone, kc, Al, Mp, Kp, L, top));
         else if (g.ka == EK_F32) MODP_TRY(split_a_t<float>(ctx, g, g.A, kdone, kc, Al, Mp, Kp, L, top));
         else MODP_TRY(split_a_t<int32_t>(ctx, g, g.A, kdone, kc, Al, Mp, Kp, L, top));
@@ -412,6 +424,25 @@ int gemm_plain_f64(modp_ctx *ctx, const GemmArgs &g)
 int gemm_limb_i8(modp_ctx *ctx, const GemmArgs &g)
 {
     return gemm_limb_cfg(ctx, g, gemm_default_config(ctx, g, MODP_FORM_LIMB_I8));
+}
+
+size_t gemm_limb_scratch_bytes(const modp_ctx *ctx, const GemmArgs &g)
+{
+    GemmConfig cfg = gemm_default_config(ctx, g, MODP_FORM_LIMB_I8);
+    const int L = limb_count(g.mp.p);
+    const int tm = cfg.tile_m == 128 ? 128 : 64;
+    const int tn = (cfg.tile_m == 128 && cfg.tile_n == 128 && L <= 2) ? 128 : 64;
+    const int64_t kmax = ((int64_t)2147483647 / ((int64_t)L * 16384)) / LBK * LBK;
+    const int64_t kc = g.k < kmax ? g.k : kmax;
+    const int64_t Kp = round_up(kc > 0 ? kc : 1, LBK);
+    return (size_t)L * (size_t)(round_up(g.m, tm) + round_up(g.n, tn)) * (size_t)Kp;
+}
+
+int gemm_limb_i8_in(modp_ctx *ctx, const GemmArgs &g, const LimbScratch &ws)
+{
+    GemmConfig cfg = gemm_default_config(ctx, g, MODP_FORM_LIMB_I8);
+    cfg.ksplit = 1;
+    return gemm_limb_cfg(ctx, g, cfg, &ws);
 }
 
 int gemv_run(modp_ctx *ctx, int kind, int64_t m, int64_t k, const void *A, int64_t lda, const void *x, void *y,

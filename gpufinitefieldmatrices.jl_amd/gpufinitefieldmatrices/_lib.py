@@ -82,6 +82,7 @@ SIGNATURES = [
     ("modp_rand", _int, [_vp, _int, _i64, _i64, _vp, _i64, _i64, _u64, _u64]),
     ("modp_eye", _int, [_vp, _int, _i64, _i64, _vp, _i64]),
     ("modp_gemm", _int, [_vp, _int, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64]),
+    ("modp_gemm_noalloc", _int, [_vp, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64]),
     ("modp_gemv", _int, [_vp, _int, _i64, _i64, _vp, _i64, _vp, _vp, _i64]),
     ("modp_autotune_mode", _int, [_vp, _int]),
     ("modp_autotune_gemm", _int, [_vp, _int, _i64, _i64, _i64, _i64, ctypes.POINTER(GemmConfig),

@@ -111,6 +111,22 @@ int modp_gemm(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, int64_t 
               const void *A, int64_t lda, const void *B, int64_t ldb,
               void *C, int64_t ldc, int64_t p);
 
+/* modp_gemm in the limb form that allocates nothing: C = A*B mod p, with the
+ * int8 digit planes of A and B held in a static device array of 64 MiB that
+ * the library owns (one per device, kept until the process ends), so the
+ * call makes no hipMalloc/hipFree or other allocating call.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  The
+ * planes take L * (m' + n') * k' bytes (L digits of p, m' and n' rounded up to
+ * the output tile, k' = min(k, 32704) rounded up to 64); larger products
+ * return MODP_ERR_ARG.  Every call on a device shares that device's array:
+ * calls that overlap on one device (two streams, two contexts, two threads)
+ * race and must be serialised by the caller.  Kernel timing hooks are not
+ * recorded for these calls. */
+int modp_gemm_noalloc(modp_ctx *ctx, int dtype, int64_t m, int64_t n, int64_t k,
+                      const void *A, int64_t lda, const void *B, int64_t ldb,
+                      void *C, int64_t ldc, int64_t p);
+
 /* y = A*x mod p: A m x k (column-major, leading dimension lda), x (k) and y
  * (m) contiguous device vectors; entries reduced mod p on load.  Exact for
  * 2 <= p <= 2^26.  Replaces stripe_mul!(z,A,x) (stripe_mul.jl:82-168) and
