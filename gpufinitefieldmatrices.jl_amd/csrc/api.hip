@@ -383,12 +383,14 @@ int modp_gemm(modp_ctx *ctx, int form, int dtype, int64_t m, int64_t n, int64_t 
     CTX_CHECK(ctx);
     GemmArgs g;
     MODP_TRY(gemm_prepare(ctx, dtype, m, n, k, A, lda, B, ldb, C, ldc, p, g));
-    if (form != MODP_FORM_AUTO && form != MODP_FORM_PLAIN_F64 && form != MODP_FORM_LIMB_I8)
+    if (form != MODP_FORM_AUTO && form != MODP_FORM_PLAIN_F64 && form != MODP_FORM_LIMB_I8 &&
+        form != MODP_FORM_RNS_I8)
         return set_error(ctx, MODP_ERR_ARG, "unknown form %d", form);
     if (m == 0 || n == 0) return MODP_OK;
     if (k == 0) return gemm_zero(ctx, g);
     if (form == MODP_FORM_AUTO) return gemm_auto(ctx, g);
     if (form == MODP_FORM_LIMB_I8) return gemm_limb_i8(ctx, g);
+    if (form == MODP_FORM_RNS_I8) return gemm_rns_i8(ctx, g);
     return gemm_plain_f64(ctx, g);
 }
 
@@ -628,11 +630,9 @@ int modp_gemm_batched(modp_ctx *ctx, int dtype, const modp_gemm_job *jobs, int64
     return st != MODP_OK ? st : st2;
 }
 
-int modp_elim_batched(modp_ctx *ctx, int op, int dtype, const modp_elim_job *jobs, int64_t count)
+static int check_elim_jobs(modp_ctx *ctx, int op, int dtype, const modp_elim_job *jobs, int64_t count, int *kind)
 {
-    CTX_CHECK(ctx);
-    int kind;
-    MODP_TRY(kind_of(ctx, dtype, &kind));
+    MODP_TRY(kind_of(ctx, dtype, kind));
     if (op != MODP_ELIM_PLUQ && op != MODP_ELIM_LU && op != MODP_ELIM_RREF)
         return set_error(ctx, MODP_ERR_ARG, "unknown elimination op %d", op);
     if (count > 0 && !jobs) return set_error(ctx, MODP_ERR_ARG, "null jobs");
@@ -647,7 +647,39 @@ int modp_elim_batched(modp_ctx *ctx, int op, int dtype, const modp_elim_job *job
         if (!j.counts || !j.row_info || (op != MODP_ELIM_RREF && !j.col_info))
             return set_error(ctx, MODP_ERR_ARG, "job %lld: null output", (long long)i);
     }
+    return MODP_OK;
+}
+
+int modp_elim_batched(modp_ctx *ctx, int op, int dtype, const modp_elim_job *jobs, int64_t count)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(check_elim_jobs(ctx, op, dtype, jobs, count, &kind));
     return batched_elim(ctx, op, kind, jobs, count);
+}
+
+int modp_elim_batched_plan(modp_ctx *ctx, int op, int dtype, const modp_elim_job *jobs, int64_t count,
+                           modp_elim_plan **out)
+{
+    CTX_CHECK(ctx);
+    if (!out) return set_error(ctx, MODP_ERR_ARG, "null output");
+    *out = nullptr;
+    int kind;
+    MODP_TRY(check_elim_jobs(ctx, op, dtype, jobs, count, &kind));
+    return batched_elim_plan(ctx, op, kind, jobs, count, out);
+}
+
+int modp_elim_batched_run(modp_ctx *ctx, modp_elim_plan *plan)
+{
+    CTX_CHECK(ctx);
+    if (!plan) return set_error(ctx, MODP_ERR_ARG, "null plan");
+    return batched_elim_run(ctx, plan);
+}
+
+int modp_elim_batched_destroy(modp_ctx *ctx, modp_elim_plan *plan)
+{
+    CTX_CHECK(ctx);
+    return batched_elim_destroy(ctx, plan);
 }
 
 // ---- timing ----
@@ -758,6 +790,19 @@ int modp_gemv(modp_ctx *ctx, int dtype, int64_t m, int64_t k, const void *A, int
     if (p > ((int64_t)1 << 26))
         return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld above 2^26", (long long)p);
     return gemv_run(ctx, kind, m, k, A, lda, x, y, p);
+}
+
+int modp_crt(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, int r, const void *const *X,
+             const int64_t *ldx, const int64_t *p, int sign, uint32_t *Y, int64_t ldy, int64_t words)
+{
+    CTX_CHECK(ctx);
+    int kind;
+    MODP_TRY(kind_of(ctx, dtype, &kind));
+    if (r < 1 || r > 12 || !X || !ldx || !p) return set_error(ctx, MODP_ERR_ARG, "CRT needs 1..12 residue matrices");
+    for (int i = 0; i < r; ++i) MODP_TRY(check_mat(ctx, "X", X[i], rows, cols, ldx[i]));
+    if (rows > 0 && cols > 0 && (!Y || ldy < rows))
+        return set_error(ctx, MODP_ERR_ARG, "CRT output: null pointer or leading dimension below the rows");
+    return util_crt(ctx, kind, rows, cols, r, X, ldx, p, sign, Y, ldy, words);
 }
 
 int modp_crt2(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, const void *X1, int64_t ld1, int64_t p1,

@@ -6,8 +6,9 @@
 // [queue reset kernel, persistent kernel] in a hipGraph; every workgroup pulls
 // tile indices from an atomic counter until the queue is empty.
 //
-// Elimination: one workgroup per instance (m, n <= 128) with the matrix held
-// in LDS, the same pivot rule as the single-matrix path, one pass per pivot.
+// Elimination: one wavefront per instance (m, n <= 128) with the matrix held
+// in LDS, the same pivot rule as the single-matrix path; a plan uploads the
+// job table once and captures the persistent kernel in a hipGraph.
 #include <algorithm>
 #include <map>
 #include <numeric>
@@ -302,13 +303,8 @@ int batched_gemm_destroy(modp_ctx *ctx, modp_gemm_plan *pl)
 }
 
 // ---------------------------------------------------------------------------
-// batched elimination
+// batched elimination: one wavefront per instance, the matrix in LDS
 // ---------------------------------------------------------------------------
-
-namespace {
-
-constexpr int EMAX = 128;
-constexpr int EP = EMAX + 1;  // LDS row pitch (odd: column walks hit distinct banks)
 
 struct EJob {
     const void *A; int64_t lda;
@@ -318,243 +314,444 @@ struct EJob {
     int64_t *col_info;
     int64_t *counts;
     int m, n;
+    int pitch;  // LDS row pitch in elements
+    int cls;    // LDS element: 0 = 8-bit (p < 2^8), 1 = 16-bit (p < 2^16), 2 = 32-bit
     ModP mp;
 };
 
-struct ESmem {
-    uint32_t S[EMAX * EP];
-    uint32_t mult[EMAX];
-    uint32_t pivval[EMAX];
-    int pivcol[EMAX];
-    int ipiv[EMAX];
-    uint32_t wv[4];
-    int wi[4];
-    int flag;
-    int item;
+struct modp_elim_plan {
+    int op = 0, kind = EK_F64;
+    int64_t count = 0;
+    EJob *djobs = nullptr;
+    int *dcounter = nullptr;
+    size_t slot = 0;  // LDS bytes per workgroup (one wavefront)
+    int bk_off = 0;   // offset of the pivot bookkeeping in the slot
+    int grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
-// argmax of S[r][col] for r in [row, m): largest value, lowest row; result
-// in (*V, *I) for every thread.  Two barriers.
-__device__ __forceinline__ void col_argmax(ESmem &s, int m, int row, int col, uint32_t *V, int *I)
+namespace {
+
+constexpr int EMAX = 128;
+constexpr int EBOOK = EMAX * (4 + 2 + 2);  // pivot values, pivot columns, row swaps
+
+// Every lane of the wavefront works on one instance, so LDS writes by one
+// lane are read by another in program order; the compiler must not move LDS
+// accesses across these points (the hardware keeps a wavefront's LDS
+// operations in order).
+__device__ __forceinline__ void wave_sync()
 {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint32_t bv = 0;
-    int bi = 0x7fffffff;
-    for (int r = row + tid; r < m; r += 256) {
-        const uint32_t x = s.S[r * EP + col];
-        if (x > bv) { bv = x; bi = r; }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const uint32_t ov = __shfl_xor(bv, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-    }
-    if (lane == 0) { s.wv[wave] = bv; s.wi[wave] = bi; }
-    __syncthreads();
-    uint32_t v = s.wv[0];
-    int i = s.wi[0];
-#pragma unroll
-    for (int x = 1; x < 4; ++x)
-        if (s.wv[x] > v || (s.wv[x] == v && s.wi[x] < i)) { v = s.wv[x]; i = s.wi[x]; }
-    *V = v;
-    *I = i;
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_batched_elim(const EJob *__restrict__ jobs, int64_t njobs, int op,
-                                                      int *__restrict__ counter)
+__global__ void k_elim_counter_reset(int *counter)
 {
-    __shared__ ESmem s;
-    const int tid = threadIdx.x;
-    for (;;) {
-        if (tid == 0) s.item = atomicAdd(counter, 1);
-        __syncthreads();
-        const int it = s.item;
-        __syncthreads();
-        if (it >= njobs) break;
-        const EJob &j = jobs[it];
-        const int m = j.m, n = j.n;
-        const ModP mp = j.mp;
-        const T *A = (const T *)j.A;
-        for (int e = tid; e < m * n; e += 256) {
-            const int r = e % m, c = e / m;
-            s.S[r * EP + c] = to_res<T>(A[r + (int64_t)c * j.lda], mp);
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+// argmax keys: larger key = larger residue, then lower row (rows < 256)
+template <typename S> struct EKey {
+    typedef uint32_t T;  // residues below 2^16
+    static __device__ __forceinline__ T make(uint32_t v, int r) { return (v << 8) | (uint32_t)(255 - r); }
+    static __device__ __forceinline__ uint32_t val(T k) { return k >> 8; }
+    static __device__ __forceinline__ int row(T k) { return 255 - (int)(k & 255u); }
+};
+template <> struct EKey<uint32_t> {
+    typedef uint64_t T;
+    static __device__ __forceinline__ T make(uint32_t v, int r) { return ((uint64_t)v << 8) | (uint64_t)(255 - r); }
+    static __device__ __forceinline__ uint32_t val(T k) { return (uint32_t)(k >> 8); }
+    static __device__ __forceinline__ int row(T k) { return 255 - (int)(k & 255u); }
+};
+
+// Row update x - l pr (mod p) of one element.  For p < 2^16 (SMALL) the
+// caller passes c = p - pr (mod p), so t = x + l c < p + p^2 < 2^32 is
+// congruent; q = trunc(fl(t) fl(1/p)) is within one of floor(t/p) (the
+// relative error of the float quotient is below 3 2^-24 and t/p <= 2^16), so
+// r = t - q p lies in [-p, 2p) and two fix-ups finish it.  All products are
+// 24-bit integer multiplies.  Otherwise c = pr and the FP64 reduction runs
+// (|x - l pr| < 2^52 is exact).
+template <bool SMALL>
+__device__ __forceinline__ uint32_t row_update(uint32_t x, uint32_t l, uint32_t c, const ModP &mp)
+{
+    if (SMALL) {
+        const uint32_t t = __umul24(l, c) + x;
+        const uint32_t q = (uint32_t)((float)t * mp.pinvf);
+        int32_t r = (int32_t)(t - __umul24(q, mp.p32));
+        r += r < 0 ? (int32_t)mp.p32 : 0;
+        r -= r >= (int32_t)mp.p32 ? (int32_t)mp.p32 : 0;
+        return (uint32_t)r;
+    }
+    return (uint32_t)dmod(fma(-(double)l, (double)c, (double)x), mp);
+}
+
+// a b mod p without the run-time modulus-size branch of mulmod
+template <bool SMALL> __device__ __forceinline__ uint32_t mulmod_c(uint32_t a, uint32_t b, const ModP &mp)
+{
+    return SMALL ? row_update<true>(0u, a, b, mp) : (uint32_t)dmod((double)a * (double)b, mp);
+}
+
+// Rows i0..m-1 (but the pivot row) of the rank-1 update by the pivot row:
+// x_rc -= l_r pr_c over this lane's columns c0 (and c1 when WIDE).  Rows
+// whose multiplier is zero are skipped (a scalar branch); the multipliers of
+// eight rows are taken first, then all their loads are issued, then the
+// stores.
+template <typename S, bool SMALL, bool WIDE>
+__device__ __forceinline__ void update_rows(S *M, int pitch, int i0, int m, int row, int c0, int c1, uint32_t lq0,
+                                            uint32_t lq1, const ModP &mp)
+{
+    uint32_t k0 = M[row * pitch + c0], k1 = WIDE ? (uint32_t)M[row * pitch + c1] : 0u;
+    if (SMALL) {
+        k0 = k0 ? mp.p32 - k0 : 0u;
+        k1 = k1 ? mp.p32 - k1 : 0u;
+    }
+    constexpr int RS = 8;
+    for (int i = i0; i < m; i += RS) {
+        uint32_t l[RS], x0[RS], x1[RS];
+#pragma unroll
+        for (int t = 0; t < RS; ++t) {
+            const int r = i + t;
+            const uint32_t lv = (uint32_t)__builtin_amdgcn_readlane((int)(r < 64 ? lq0 : lq1), r & 63);
+            l[t] = (r < m && r != row) ? lv : 0u;
+            x0[t] = x1[t] = 0u;
+            if (l[t]) {
+                x0[t] = M[r * pitch + c0];
+                if (WIDE) x1[t] = M[r * pitch + c1];
+            }
         }
-        __syncthreads();
-        int row = 0, col = 0, last = n - 1, nr = 0, nc = 0;
-        bool bad = false;
-        while (row < m && col < n) {
-            uint32_t V;
-            int I;
-            col_argmax(s, m, row, col, &V, &I);
-            if (V == 0) {
-                if (op != MODP_ELIM_PLUQ) { ++col; continue; }
-                // pluq: whole remaining block zero?
-                if (tid == 0) s.flag = 0;
-                __syncthreads();
-                const int rows = m - row, cols = n - col;
-                for (int e = tid; e < rows * cols; e += 256)
-                    if (s.S[(row + e % rows) * EP + col + e / rows]) s.flag = 1;
-                __syncthreads();
-                const int any = s.flag;
-                __syncthreads();
-                if (!any) {
-                    // src tail: (c, n) for c = col..n-1, applied in order
-                    for (int r = tid; r < m; r += 256)
-                        for (int c = col; c < n - 1; ++c) {
-                            uint32_t t = s.S[r * EP + c];
-                            s.S[r * EP + c] = s.S[r * EP + n - 1];
-                            s.S[r * EP + n - 1] = t;
-                        }
-                    if (tid == 0)
-                        for (int c = col; c < n; ++c) {
-                            j.col_info[2 * nc] = c + 1;
-                            j.col_info[2 * nc + 1] = n;
-                            ++nc;
-                        }
-                    __syncthreads();
-                    break;
-                }
-                for (int r = tid; r < m; r += 256) {
-                    uint32_t t = s.S[r * EP + col];
-                    s.S[r * EP + col] = s.S[r * EP + last];
-                    s.S[r * EP + last] = t;
-                }
-                if (tid == 0) {
-                    j.col_info[2 * nc] = col + 1;
-                    j.col_info[2 * nc + 1] = last + 1;
-                }
-                ++nc;
-                --last;
-                __syncthreads();
-                continue;
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < RS; ++t) {
+            const int r = i + t;
+            if (l[t]) {
+                M[r * pitch + c0] = (S)row_update<SMALL>(x0[t], l[t], k0, mp);
+                if (WIDE) M[r * pitch + c1] = (S)row_update<SMALL>(x1[t], l[t], k1, mp);
             }
-            const uint32_t inv = modinv(V, mp);
-            if (inv == 0) { bad = true; break; }
-            // swap rows row <-> I and normalise the pivot row from col on
-            for (int c = tid; c < n; c += 256) {
-                const uint32_t a = s.S[row * EP + c], b = s.S[I * EP + c];
-                s.S[I * EP + c] = a;
-                s.S[row * EP + c] = c >= col ? mulmod(b, inv, mp) : b;
-            }
-            if (tid == 0) {
-                s.pivval[row] = V;
-                s.pivcol[row] = col;
-                s.ipiv[row] = I;
-                if (op == MODP_ELIM_PLUQ && I != row) {
-                    j.row_info[2 * nr] = row + 1;
-                    j.row_info[2 * nr + 1] = I + 1;
-                }
-            }
-            if (op == MODP_ELIM_PLUQ && I != row) ++nr;
-            __syncthreads();
-            if (op == MODP_ELIM_RREF) {
-                // Gauss-Jordan: clear the pivot column in every other row
-                for (int r = tid; r < m; r += 256) s.mult[r] = r == row ? 0u : s.S[r * EP + col];
-                __syncthreads();
-                const int cols = n - col;
-                for (int e = tid; e < m * cols; e += 256) {
-                    const int r = e / cols, c = col + e % cols;
-                    const uint32_t l = s.mult[r];
-                    if (l) s.S[r * EP + c] = submod(s.S[r * EP + c], mulmod(l, s.S[row * EP + c], mp), mp);
-                }
-            } else {
-                const int rows = m - row - 1, cols = n - col - 1;
-                for (int e = tid; e < rows * cols; e += 256) {
-                    const int r = row + 1 + e / cols, c = col + 1 + e % cols;
-                    const uint32_t l = s.S[r * EP + col];
-                    if (l) s.S[r * EP + c] = submod(s.S[r * EP + c], mulmod(l, s.S[row * EP + c], mp), mp);
-                }
-            }
-            __syncthreads();
-            ++row;
-            ++col;
         }
-        const int rank = row;
-        // outputs
-        T *U = (T *)j.U;
-        for (int e = tid; e < m * n; e += 256) {
-            const int r = e % m, c = e / m;
+        wave_sync();
+    }
+}
+
+// One instance on one wavefront (64 threads): argmax over rows with the lanes
+// over rows (DPP reduction of (residue, -row) keys), row swaps,
+// normalisation and the rank-1 updates with the lanes over the columns that
+// change (the multiplier of a row is one broadcast read, the pivot row sits
+// in registers).  S is the LDS element; SMALL: p < 2^16.  Every control
+// value (m, n, row, col, the pivot) is wave-uniform, so the branches are
+// scalar.
+template <typename T, typename S, bool SMALL>
+__device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned char *esm, int bk_off)
+{
+    typedef EKey<S> K;
+    typedef typename K::T key_t;
+    S *M = reinterpret_cast<S *>(esm);
+    uint32_t *pivval = reinterpret_cast<uint32_t *>(esm + bk_off);
+    int16_t *pivcol = reinterpret_cast<int16_t *>(pivval + EMAX);
+    int16_t *ipiv = pivcol + EMAX;
+    const int lane = threadIdx.x;
+    const int m = j.m, n = j.n, pitch = j.pitch;
+    const ModP &mp = j.mp;
+    const T *A = (const T *)j.A;
+    // column-major input, lanes over rows (two rows per lane at most):
+    // 16 columns of loads in flight before their LDS stores
+    for (int c0 = 0; c0 < n; c0 += 16) {
+        T x0[16], x1[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const bool cv = c0 + t < n;
+            x0[t] = (cv && lane < m) ? A[lane + (int64_t)(c0 + t) * j.lda] : T(0);
+            x1[t] = (cv && lane + 64 < m) ? A[lane + 64 + (int64_t)(c0 + t) * j.lda] : T(0);
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            if (c0 + t < n && lane < m) M[lane * pitch + c0 + t] = (S)to_res<T>(x0[t], mp);
+            if (c0 + t < n && lane + 64 < m) M[(lane + 64) * pitch + c0 + t] = (S)to_res<T>(x1[t], mp);
+        }
+    }
+    wave_sync();
+    int row = 0, col = 0, last = n - 1, nr = 0, nc = 0;
+    bool bad = false;
+    while (row < m && col < n) {
+        // largest residue of column col in rows row..m-1, lowest row on ties
+        key_t mine = 0;
+        if (lane >= row && lane < m) mine = K::make(M[lane * pitch + col], lane);
+        if (lane + 64 >= row && lane + 64 < m) {
+            const key_t k = K::make(M[(lane + 64) * pitch + col], lane + 64);
+            mine = k > mine ? k : mine;
+        }
+        const key_t best = wave_max(mine);
+        const uint32_t bv = K::val(best);
+        if (bv == 0) {
+            if (op != MODP_ELIM_PLUQ) { ++col; continue; }
+            // pluq: is the whole block rows row.., columns col.. zero?
+            int nz = 0;
+            for (int c = col + lane; c < n; c += 64)
+                for (int r = row; r < m; ++r) nz |= M[r * pitch + c] != 0;
+            if (!__any(nz)) {
+                // the reference's tail: (c, n) for c = col..n-1 in order,
+                // which rotates columns col..n-1 right by one
+                const int ca = col + 1 + lane, cb = col + 65 + lane;
+                for (int r = 0; r < m; ++r) {
+                    const S first = M[r * pitch + n - 1];
+                    const S va = ca < n ? M[r * pitch + ca - 1] : S(0);
+                    const S vb = cb < n ? M[r * pitch + cb - 1] : S(0);
+                    wave_sync();
+                    if (ca < n) M[r * pitch + ca] = va;
+                    if (cb < n) M[r * pitch + cb] = vb;
+                    if (lane == 0) M[r * pitch + col] = first;
+                    wave_sync();
+                }
+                if (lane == 0)
+                    for (int c = col, k = nc; c < n; ++c, ++k) {
+                        j.col_info[2 * k] = c + 1;
+                        j.col_info[2 * k + 1] = n;
+                    }
+                nc += n - col;
+                break;
+            }
+            // swap with the last column not yet swapped; retry this column
+            for (int r = lane; r < m; r += 64) {
+                const S t = M[r * pitch + col];
+                M[r * pitch + col] = M[r * pitch + last];
+                M[r * pitch + last] = t;
+            }
+            if (lane == 0) {
+                j.col_info[2 * nc] = col + 1;
+                j.col_info[2 * nc + 1] = last + 1;
+            }
+            ++nc;
+            --last;
+            wave_sync();
+            continue;
+        }
+        const uint32_t inv = modinv_fast(bv, mp);
+        if (inv == 0) { bad = true; break; }
+        const int I = K::row(best);
+        // swap rows row <-> I and normalise the pivot row from col on
+        for (int c = lane; c < n; c += 64) {
+            const uint32_t a = M[row * pitch + c], b = M[I * pitch + c];
+            M[I * pitch + c] = (S)a;
+            M[row * pitch + c] = (S)(c >= col ? mulmod_c<SMALL>(b, inv, mp) : b);
+        }
+        if (lane == 0) {
+            pivval[row] = bv;
+            pivcol[row] = (int16_t)col;
+            ipiv[row] = (int16_t)I;
+            if (op == MODP_ELIM_PLUQ && I != row) {
+                j.row_info[2 * nr] = row + 1;
+                j.row_info[2 * nr + 1] = I + 1;
+            }
+        }
+        if (op == MODP_ELIM_PLUQ && I != row) ++nr;
+        wave_sync();
+        // the update covers columns lo..n-1: > col (pluq, lu) or >= col
+        // (rref, which also clears the rows above the pivot).  Lane q owns
+        // columns lo+q and lo+64+q (the second only while more than 64
+        // columns remain); lanes past the last column repeat it, computing
+        // and storing the same value as its owner, so no lane is masked off.
+        // The multipliers (column col) are read once, lane q holding rows q
+        // and q+64, and reach the update as scalars.
+        const int lo = op == MODP_ELIM_RREF ? col : col + 1;
+        if (lo < n) {
+            const int c0 = min(lo + lane, n - 1), c1 = min(lo + 64 + lane, n - 1);
+            const uint32_t lq0 = lane < m ? (uint32_t)M[lane * pitch + col] : 0u;
+            const uint32_t lq1 = lane + 64 < m ? (uint32_t)M[(lane + 64) * pitch + col] : 0u;
+            const int i0 = op == MODP_ELIM_RREF ? 0 : row + 1;
+            if (n - lo > 64)
+                update_rows<S, SMALL, true>(M, pitch, i0, m, row, c0, c1, lq0, lq1, mp);
+            else
+                update_rows<S, SMALL, false>(M, pitch, i0, m, row, c0, c1, lq0, lq1, mp);
+        }
+        wave_sync();
+        ++row;
+        ++col;
+    }
+    const int rank = row;
+    // outputs, column-major: lanes over rows write consecutive elements
+    T *U = (T *)j.U;
+    for (int c = 0; c < n; ++c)
+        for (int r = lane; r < m; r += 64) {
             uint32_t v = 0;
-            if (op == MODP_ELIM_RREF) v = r < rank ? s.S[r * EP + c] : 0u;
-            else if (r < rank && c >= s.pivcol[r]) v = s.S[r * EP + c];
+            if (op == MODP_ELIM_RREF) v = r < rank ? (uint32_t)M[r * pitch + c] : 0u;
+            else if (r < rank && c >= pivcol[r]) v = M[r * pitch + c];
             U[r + (int64_t)c * j.ldu] = from_res<T>(v);
         }
-        if (op != MODP_ELIM_RREF) {
-            T *L = (T *)j.L;
-            for (int e = tid; e < m * m; e += 256) {
-                const int r = e % m, k = e / m;
+    if (op != MODP_ELIM_RREF) {
+        T *L = (T *)j.L;
+        for (int k = 0; k < m; ++k) {
+            const int pc = k < rank ? pivcol[k] : 0;
+            const uint32_t pv = k < rank ? pivval[k] : 0u;
+            for (int r = lane; r < m; r += 64) {
                 uint32_t v = 0;
-                if (k < rank) {
-                    if (r == k) v = s.pivval[k];
-                    else if (r > k) v = s.S[r * EP + s.pivcol[k]];
-                }
+                if (k < rank) v = r == k ? pv : (r > k ? (uint32_t)M[r * pitch + pc] : 0u);
                 L[r + (int64_t)k * j.ldl] = from_res<T>(v);
             }
         }
-        if (tid == 0) {
-            if (op == MODP_ELIM_LU) {
-                for (int i = 0; i < m; ++i) j.row_info[i] = i + 1;
-                for (int i = 0; i < rank; ++i) {
-                    const int64_t t = j.row_info[i];
-                    j.row_info[i] = j.row_info[s.ipiv[i]];
-                    j.row_info[s.ipiv[i]] = t;
-                }
-                for (int i = 0; i < rank; ++i) j.col_info[i] = s.pivcol[i] + 1;
-            } else if (op == MODP_ELIM_RREF) {
-                for (int i = 0; i < rank; ++i) j.row_info[i] = s.pivcol[i] + 1;
-            }
-            j.counts[0] = op == MODP_ELIM_PLUQ ? nr : 0;
-            j.counts[1] = op == MODP_ELIM_PLUQ ? nc : 0;
-            j.counts[2] = bad ? -1 : rank;
-        }
-        __syncthreads();
     }
+    if (lane == 0) {
+        if (op == MODP_ELIM_LU) {
+            for (int q = 0; q < m; ++q) j.row_info[q] = q + 1;
+            for (int q = 0; q < rank; ++q) {
+                const int64_t t = j.row_info[q];
+                j.row_info[q] = j.row_info[ipiv[q]];
+                j.row_info[ipiv[q]] = t;
+            }
+            for (int q = 0; q < rank; ++q) j.col_info[q] = pivcol[q] + 1;
+        } else if (op == MODP_ELIM_RREF) {
+            for (int q = 0; q < rank; ++q) j.row_info[q] = pivcol[q] + 1;
+        }
+        j.counts[0] = op == MODP_ELIM_PLUQ ? nr : 0;
+        j.counts[1] = op == MODP_ELIM_PLUQ ? nc : 0;
+        j.counts[2] = bad ? -1 : rank;
+    }
+    wave_sync();
+}
+
+// Persistent kernel: each single-wavefront workgroup takes instances from
+// one device queue (the job table is sorted by cost, largest first), the
+// next index fetched while the current instance runs.
+template <typename T>
+__global__ __launch_bounds__(64) void k_elim_wave(const EJob *__restrict__ jobs, int64_t njobs, int op,
+                                                  int *__restrict__ counter, int bk_off)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char esm[];
+    int it = 0;
+    if (threadIdx.x == 0) it = atomicAdd(counter, 1);
+    it = __builtin_amdgcn_readfirstlane(it);
+    while (it < njobs) {
+        int next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        const EJob j = jobs[it];
+        if (j.cls == 0) elim_instance<T, uint8_t, true>(j, op, esm, bk_off);
+        else if (j.cls == 1) elim_instance<T, uint16_t, true>(j, op, esm, bk_off);
+        else elim_instance<T, uint32_t, false>(j, op, esm, bk_off);
+        it = __builtin_amdgcn_readfirstlane(next);
+    }
+}
+
+template <typename T> hipError_t allow_elim_slot(size_t slot)
+{
+    // the dynamic LDS can exceed the default 64 KiB (32-bit residues of
+    // 128 x 128 instances)
+    return hipFuncSetAttribute((const void *)k_elim_wave<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slot);
+}
+
+template <typename T> hipError_t launch_elim(const modp_elim_plan *pl, hipStream_t st)
+{
+    k_elim_wave<T><<<(unsigned)pl->grid, 64, pl->slot, st>>>(pl->djobs, pl->count, pl->op, pl->dcounter, pl->bk_off);
+    return hipGetLastError();
+}
+
+int elim_class(int64_t p) { return p < 256 ? 0 : (p < 65536 ? 1 : 2); }
+
+// a row pitch of an odd number of 32-bit words, so the lanes of a column
+// walk hit distinct banks
+int elim_pitch(int n, int eb)
+{
+    int pitch = n;
+    while ((pitch * eb) % 4 != 0 || ((pitch * eb) / 4) % 2 == 0) ++pitch;
+    return pitch;
 }
 
 }  // namespace
 
-int batched_elim(modp_ctx *ctx, int op, int kind, const modp_elim_job *jobs, int64_t count)
+int batched_elim_destroy(modp_ctx *ctx, modp_elim_plan *pl)
 {
-    if (count <= 0) return MODP_OK;
+    if (!pl) return MODP_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    if (pl->djobs) (void)hipFree(pl->djobs);
+    if (pl->dcounter) (void)hipFree(pl->dcounter);
+    delete pl;
+    return MODP_OK;
+}
+
+// The plan sorts the instances by cost (largest first, so the queue ends
+// with short ones), gives each its LDS element and row pitch, sizes the LDS
+// slot for the largest, uploads the job table once and captures
+// [queue reset, one persistent kernel] in a hipGraph.
+int batched_elim_plan(modp_ctx *ctx, int op, int kind, const modp_elim_job *jobs, int64_t count,
+                      modp_elim_plan **out)
+{
     if (kind == EK_I32) return set_error(ctx, MODP_ERR_DTYPE, "batched elimination: unsupported element type");
-    std::vector<EJob> hj((size_t)count);
-    std::vector<int64_t> order((size_t)count);
-    std::iota(order.begin(), order.end(), 0);
+    modp_elim_plan *pl = new modp_elim_plan();
+    pl->op = op;
+    pl->kind = kind;
+    pl->count = count;
+    std::vector<EJob> hj;
+    hj.reserve((size_t)(count > 0 ? count : 0));
+    // cost: multiply-adds of the updates, the 32-bit class at twice the rate
     auto cost = [&](int64_t i) {
         const int64_t a = jobs[i].m, b = jobs[i].n;
-        return a * b * (a < b ? a : b);
+        return a * b * (a < b ? a : b) * (elim_class(jobs[i].p) == 2 ? 2 : 1);
     };
+    std::vector<int64_t> order((size_t)(count > 0 ? count : 0));
+    std::iota(order.begin(), order.end(), (int64_t)0);
     std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
-    for (int64_t i = 0; i < count; ++i) {
-        const modp_elim_job &s = jobs[order[i]];
-        EJob &d = hj[i];
+    size_t mat_bytes = 16;
+    for (int64_t oi : order) {
+        const modp_elim_job &s = jobs[oi];
+        EJob d;
         d.A = s.A; d.lda = s.lda; d.U = s.U; d.ldu = s.ldu; d.L = s.L; d.ldl = s.ldl;
         d.row_info = s.row_info; d.col_info = s.col_info; d.counts = s.counts;
         d.m = (int)s.m; d.n = (int)s.n;
+        d.cls = elim_class(s.p);
+        const int eb = d.cls == 0 ? 1 : (d.cls == 1 ? 2 : 4);
+        d.pitch = elim_pitch(d.n, eb);
         d.mp = make_modp(s.p);
+        mat_bytes = std::max(mat_bytes, (size_t)d.m * (size_t)d.pitch * (size_t)eb);
+        hj.push_back(d);
     }
-    int st = MODP_OK;
-    char *buf = (char *)ws_get(ctx, WS_BATCH, sizeof(EJob) * count + 256, &st);
-    if (!buf) return st;
-    int *counter = (int *)buf;
-    EJob *dj = (EJob *)(buf + 256);
-    HIP_TRY(ctx, hipMemcpyAsync(dj, hj.data(), sizeof(EJob) * count, hipMemcpyHostToDevice, ctx->stream));
-    k_queue_reset<<<1, 64, 0, ctx->stream>>>(counter);
-    HIP_TRY(ctx, hipGetLastError());
-    int64_t g = (int64_t)ctx->num_cus * 2;
+    pl->bk_off = (int)((mat_bytes + 15) & ~(size_t)15);
+    pl->slot = (size_t)pl->bk_off + EBOOK;
+    // persistent grid: as many single-wavefront workgroups as the LDS holds
+    const size_t lds_cu = 160 * 1024;
+    int64_t per_cu = (int64_t)(lds_cu / pl->slot);
+    if (per_cu > 16) per_cu = 16;
+    if (per_cu < 1) per_cu = 1;
+    int64_t g = per_cu * ctx->num_cus;
     if (g > count) g = count;
-    {
-        TimedLaunch tl(ctx, "batched_elim");
-        if (kind == EK_F64)
-            k_batched_elim<double><<<(unsigned)g, 256, 0, ctx->stream>>>(dj, count, op, counter);
-        else
-            k_batched_elim<float><<<(unsigned)g, 256, 0, ctx->stream>>>(dj, count, op, counter);
-        HIP_TRY(ctx, hipGetLastError());
+    pl->grid = (int)(g > 0 ? g : 1);
+    hipError_t e = hipMalloc(&pl->djobs, sizeof(EJob) * (count > 0 ? count : 1));
+    if (e == hipSuccess) e = hipMalloc(&pl->dcounter, 256);
+    if (e == hipSuccess && count > 0)
+        e = hipMemcpyAsync(pl->djobs, hj.data(), sizeof(EJob) * count, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // hj is released on return
+    if (e == hipSuccess) e = kind == EK_F64 ? allow_elim_slot<double>(pl->slot) : allow_elim_slot<float>(pl->slot);
+    if (e == hipSuccess) e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        k_elim_counter_reset<<<1, 64, 0, ctx->stream>>>(pl->dcounter);
+        const hipError_t le = kind == EK_F64 ? launch_elim<double>(pl, ctx->stream) : launch_elim<float>(pl, ctx->stream);
+        e = hipStreamEndCapture(ctx->stream, &pl->graph);
+        if (e == hipSuccess) e = le;
     }
-    // the job table lives in a reused workspace: finish before returning
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (e == hipSuccess) e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        const int st = hip_check(ctx, e, "batched elimination plan");
+        batched_elim_destroy(ctx, pl);
+        return st;
+    }
+    *out = pl;
     return MODP_OK;
+}
+
+int batched_elim_run(modp_ctx *ctx, modp_elim_plan *pl)
+{
+    if (pl->count == 0) return MODP_OK;
+    TimedLaunch tl(ctx, "batched_elim");
+    return hip_check(ctx, hipGraphLaunch(pl->exec, ctx->stream), "hipGraphLaunch");
+}
+
+int batched_elim(modp_ctx *ctx, int op, int kind, const modp_elim_job *jobs, int64_t count)
+{
+    if (count <= 0) return MODP_OK;
+    modp_elim_plan *pl = nullptr;
+    MODP_TRY(batched_elim_plan(ctx, op, kind, jobs, count, &pl));
+    int st = batched_elim_run(ctx, pl);
+    // the plan owns the job table: finish before releasing it
+    if (st == MODP_OK) st = hip_check(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    batched_elim_destroy(ctx, pl);
+    return st;
 }

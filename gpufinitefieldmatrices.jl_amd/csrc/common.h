@@ -123,6 +123,68 @@ __device__ __forceinline__ bool tile_of_block(int64_t tiles_m, int64_t tiles_n, 
 
 static inline unsigned xcd_grid(int64_t tiles) { return (unsigned)(((tiles + 7) / 8) * 8); }
 
+// a^{-1} mod p for p < 2^24: extended Euclid whose quotients come from a
+// float reciprocal (exact operands below 2^24, estimate corrected to the
+// exact quotient); larger p use the double-division variant.
+__device__ __forceinline__ uint32_t modinv_fast(uint32_t a, const ModP &mp)
+{
+    if (mp.p >= (1 << 24)) return modinv(a, mp);
+    int32_t inv = 0, ninv = 1, rem = (int32_t)mp.p32, nrem = (int32_t)a;
+    while (nrem != 0) {
+        int32_t q = (int32_t)((float)rem * __builtin_amdgcn_rcpf((float)nrem));
+        int32_t r = rem - q * nrem;
+        while (r < 0) { --q; r += nrem; }
+        while (r >= nrem) { ++q; r -= nrem; }
+        int32_t t = inv - q * ninv; inv = ninv; ninv = t;
+        rem = nrem; nrem = r;
+    }
+    if (rem != 1) return 0;
+    if (inv < 0) inv += (int32_t)mp.p32;
+    return (uint32_t)inv;
+}
+
+// ---------------------------------------------------------------------------
+// wavefront reductions by DPP
+// ---------------------------------------------------------------------------
+
+template <int CTRL> __device__ __forceinline__ uint32_t dpp32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL> __device__ __forceinline__ uint64_t dpp64(uint64_t v)
+{
+    return ((uint64_t)dpp32<CTRL>((uint32_t)(v >> 32)) << 32) | dpp32<CTRL>((uint32_t)v);
+}
+template <typename T> __device__ __forceinline__ T kmax(T a, T b) { return a > b ? a : b; }
+
+// wave64 maximum: DPP within each row of 16 lanes (quad swaps, half-row and
+// row mirrors), then the four row results through readlane
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+    v = kmax(v, dpp32<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = kmax(v, dpp32<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = kmax(v, dpp32<0x141>(v));  // row_half_mirror
+    v = kmax(v, dpp32<0x140>(v));  // row_mirror
+    const uint32_t r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const uint32_t r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return kmax(kmax(r0, r1), kmax(r2, r3));
+}
+__device__ __forceinline__ uint64_t wave_max(uint64_t v)
+{
+    v = kmax(v, dpp64<0xB1>(v));
+    v = kmax(v, dpp64<0x4E>(v));
+    v = kmax(v, dpp64<0x141>(v));
+    v = kmax(v, dpp64<0x140>(v));
+    // readlane returns a signed int: both halves go through uint32_t so a low
+    // word with bit 31 set is not sign-extended into the high word
+    uint64_t r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        r[i] = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), 16 * i) << 32) |
+               (uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)v, 16 * i);
+    return kmax(kmax(r[0], r[1]), kmax(r[2], r[3]));
+}
+
 // storage element -> residue in [0,p)
 template <typename T> __device__ __forceinline__ uint32_t to_res(T x, const ModP &mp);
 template <> __device__ __forceinline__ uint32_t to_res<double>(double x, const ModP &mp)
@@ -293,6 +355,7 @@ struct GemmArgs {
 // heuristic configurations (gemm.hip)
 int gemm_plain_f64(modp_ctx *ctx, const GemmArgs &g);
 int gemm_limb_i8(modp_ctx *ctx, const GemmArgs &g);
+int gemm_rns_i8(modp_ctx *ctx, const GemmArgs &g);
 // the limb form with its digit planes in a caller's workspace (no allocation,
 // one K slice); gemm_limb_scratch_bytes is the workspace that product needs
 struct LimbScratch {
@@ -300,6 +363,12 @@ struct LimbScratch {
     size_t bytes;
 };
 size_t gemm_limb_scratch_bytes(const modp_ctx *ctx, const GemmArgs &g);
+// the multi-modular (RNS) int8 form (gemm_rns.hip): moduli needed for an
+// exact k-deep product mod p (0: none suffice), and the product with a
+// tile x tile output tile
+int rns_count(int64_t p, int64_t k);
+int gemm_rns_run(modp_ctx *ctx, const GemmArgs &g, int tile);
+constexpr int64_t RNS_KCHUNK = 130944;
 int gemm_limb_i8_in(modp_ctx *ctx, const GemmArgs &g, const LimbScratch &ws);
 int limb_count(int64_t p);
 int choose_form(int64_t m, int64_t n, int64_t k, int64_t p);
@@ -328,6 +397,8 @@ int util_convert(modp_ctx *ctx, int kin, int64_t rows, int64_t cols, const void 
                  int kout, void *Y, int64_t ldy, const ModP &mp);
 int util_perm(modp_ctx *ctx, int kind, int rowwise, int64_t rows, int64_t cols, void *X, int64_t ldx,
               const int64_t *stack_host, int64_t nswaps, int inverse);
+int util_crt(modp_ctx *ctx, int kind, int64_t rows, int64_t cols, int r, const void *const *X, const int64_t *ldx,
+             const int64_t *p, int sign, uint32_t *Y, int64_t ldy, int64_t words);
 int util_crt2(modp_ctx *ctx, int kind, int64_t rows, int64_t cols, const void *X1, int64_t ld1, int64_t p1,
               const void *X2, int64_t ld2, int64_t p2, void *Y, int64_t ldy);
 

@@ -87,26 +87,6 @@ template <int VW> __device__ __forceinline__ void store_rows(int32_t *p, const i
     }
 }
 
-// a^{-1} mod p for p < 2^24: extended Euclid whose quotients come from a
-// float reciprocal (exact operands below 2^24, estimate corrected to the
-// exact quotient); larger p use the double-division variant.
-__device__ __forceinline__ uint32_t modinv_fast(uint32_t a, const ModP &mp)
-{
-    if (mp.p >= (1 << 24)) return modinv(a, mp);
-    int32_t inv = 0, ninv = 1, rem = (int32_t)mp.p32, nrem = (int32_t)a;
-    while (nrem != 0) {
-        int32_t q = (int32_t)((float)rem * __builtin_amdgcn_rcpf((float)nrem));
-        int32_t r = rem - q * nrem;
-        while (r < 0) { --q; r += nrem; }
-        while (r >= nrem) { ++q; r -= nrem; }
-        int32_t t = inv - q * ninv; inv = ninv; ninv = t;
-        rem = nrem; nrem = r;
-    }
-    if (rem != 1) return 0;
-    if (inv < 0) inv += (int32_t)mp.p32;
-    return (uint32_t)inv;
-}
-
 template <bool SMALL> __device__ __forceinline__ uint32_t mulmod_t(uint32_t a, uint32_t b, const ModP &mp)
 {
     if (SMALL) {
@@ -147,42 +127,6 @@ template <> struct PivKey<false> {
     static __device__ __forceinline__ uint32_t val(T k) { return (uint32_t)(k >> 32); }
     static __device__ __forceinline__ int pos(T k) { return 0x7fffffff - (int)(uint32_t)k; }
 };
-
-template <int CTRL> __device__ __forceinline__ uint32_t dpp32(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
-}
-template <int CTRL> __device__ __forceinline__ uint64_t dpp64(uint64_t v)
-{
-    return ((uint64_t)dpp32<CTRL>((uint32_t)(v >> 32)) << 32) | dpp32<CTRL>((uint32_t)v);
-}
-template <typename T> __device__ __forceinline__ T kmax(T a, T b) { return a > b ? a : b; }
-
-// wave64 maximum: DPP within each row of 16 lanes (quad swaps, half-row and
-// row mirrors), then the four row results through readlane
-__device__ __forceinline__ uint32_t wave_max(uint32_t v)
-{
-    v = kmax(v, dpp32<0xB1>(v));   // quad_perm [1,0,3,2]
-    v = kmax(v, dpp32<0x4E>(v));   // quad_perm [2,3,0,1]
-    v = kmax(v, dpp32<0x141>(v));  // row_half_mirror
-    v = kmax(v, dpp32<0x140>(v));  // row_mirror
-    const uint32_t r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
-    const uint32_t r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
-    return kmax(kmax(r0, r1), kmax(r2, r3));
-}
-__device__ __forceinline__ uint64_t wave_max(uint64_t v)
-{
-    v = kmax(v, dpp64<0xB1>(v));
-    v = kmax(v, dpp64<0x4E>(v));
-    v = kmax(v, dpp64<0x141>(v));
-    v = kmax(v, dpp64<0x140>(v));
-    uint64_t r[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        r[i] = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), 16 * i) << 32) |
-               __builtin_amdgcn_readlane((uint32_t)v, 16 * i);
-    return kmax(kmax(r[0], r[1]), kmax(r[2], r[3]));
-}
 
 // Panel factorisation: one 512-thread workgroup, the panel W[kr0:m, c0:c0+w]
 // in registers (thread t owns local rows t + 512*q) as exact integers in

@@ -373,6 +373,10 @@ bool gemm_config_valid(const GemmArgs &g, const GemmConfig &cfg)
     const bool sq128 = cfg.tile_m == 128 && cfg.tile_n == 128;
     const bool r128x64 = cfg.tile_m == 128 && cfg.tile_n == 64;
     if (cfg.form == MODP_FORM_PLAIN_F64) return (sq64 || sq128) && f64_reduce_interval(g.mp.p) >= 1;
+    if (cfg.form == MODP_FORM_RNS_I8) {
+        const bool sq256 = cfg.tile_m == 256 && cfg.tile_n == 256;
+        return (sq128 || sq256) && cfg.ksplit == 1 && rns_count(g.mp.p, g.k < RNS_KCHUNK ? g.k : RNS_KCHUNK) > 0;
+    }
     // 128x128 limb tiles hold 2L-1 accumulators of a 64x64 block per wave:
     // register room for L <= 2 only
     if (cfg.form == MODP_FORM_LIMB_I8) {
@@ -382,10 +386,28 @@ bool gemm_config_valid(const GemmArgs &g, const GemmConfig &cfg)
     return false;
 }
 
+// the RNS form's r moduli against the limb form's L^2 digit products: it
+// pays when it issues at most three quarters of the limb form's MFMA work
+static bool rns_pays(const GemmArgs &g)
+{
+    const int L = limb_count(g.mp.p);
+    const int r = rns_count(g.mp.p, g.k < RNS_KCHUNK ? g.k : RNS_KCHUNK);
+    return r > 0 && 4 * r <= 3 * L * L;
+}
+
 GemmConfig gemm_default_config(const modp_ctx *ctx, const GemmArgs &g, int form)
 {
     GemmConfig c;
     c.form = form == MODP_FORM_AUTO ? choose_form(g.m, g.n, g.k, g.mp.p) : form;
+    if (form == MODP_FORM_AUTO && c.form == MODP_FORM_LIMB_I8 && rns_pays(g)) c.form = MODP_FORM_RNS_I8;
+    c.ksplit = 1;
+    if (c.form == MODP_FORM_RNS_I8) {
+        // 256x256 tiles while the grid (tiles x moduli) still holds two
+        // workgroups per CU
+        const int r = rns_count(g.mp.p, g.k < RNS_KCHUNK ? g.k : RNS_KCHUNK);
+        c.tile_m = c.tile_n = tiles_of(g, 256, 256) * r >= 2 * ctx->num_cus ? 256 : 128;
+        return c;
+    }
     c.tile_m = c.tile_n = 64;
     // larger tiles while the grid still fills the chip: measured best at
     // n = 8192 are 128x128 (plain; limb with one digit) and 128x64 (limb with
@@ -413,6 +435,7 @@ GemmConfig gemm_default_config(const modp_ctx *ctx, const GemmArgs &g, int form)
 int gemm_run(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg)
 {
     if (cfg.form == MODP_FORM_LIMB_I8) return gemm_limb_cfg(ctx, g, cfg);
+    if (cfg.form == MODP_FORM_RNS_I8) return gemm_rns_run(ctx, g, cfg.tile_m);
     return gemm_plain_cfg(ctx, g, cfg);
 }
 
@@ -424,6 +447,11 @@ int gemm_plain_f64(modp_ctx *ctx, const GemmArgs &g)
 int gemm_limb_i8(modp_ctx *ctx, const GemmArgs &g)
 {
     return gemm_limb_cfg(ctx, g, gemm_default_config(ctx, g, MODP_FORM_LIMB_I8));
+}
+
+int gemm_rns_i8(modp_ctx *ctx, const GemmArgs &g)
+{
+    return gemm_run(ctx, g, gemm_default_config(ctx, g, MODP_FORM_RNS_I8));
 }
 
 size_t gemm_limb_scratch_bytes(const modp_ctx *ctx, const GemmArgs &g)

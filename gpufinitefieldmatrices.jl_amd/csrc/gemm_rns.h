@@ -1,0 +1,549 @@
+// gemm_rns.h -- multi-modular (residue number system) form of the mod-p GEMM
+// on gfx950 int8 MFMA.
+//
+// C = A*B mod p is computed through the exact integer product of the
+// balanced representatives (|a|, |b| <= floor(p/2)):  C_int = sum_k a b,
+// |C_int| <= k floor(p/2)^2.  That integer is taken modulo r pairwise coprime
+// moduli q_0..q_{r-1} <= 256 with M = prod q_j > 2 |C_int|:
+//
+//   * split: every operand entry is reduced mod each q_j into a balanced
+//     residue in [-128, 127] held in int8 (one plane per modulus); the B
+//     side is multiplied by y_j = (M/q_j)^{-1} mod q_j on the way;
+//   * core: one int8 GEMM per modulus on v_mfma_i32_32x32x32_i8 with int32
+//     accumulation (|partial sums| <= k 2^14 < 2^31 for k <= 131071), whose
+//     epilogue writes e_j = D_j mod q_j = (y_j C_int) mod q_j as one byte;
+//   * combine: C_int = sum_j e_j M/q_j - t M with t = round(sum_j e_j / q_j)
+//     (exact: |C_int| / M < 1/2 leaves a margin far above the FP64 error of
+//     the sum), so C mod p = (sum_j e_j (M/q_j mod p) - t (M mod p)) mod p,
+//     every term an exact integer below 2^40 in FP64.
+//
+// This is the Chinese-remainder reconstruction (Garner 1959; Knuth TAOCP
+// vol. 2 4.3.2) applied to an integer matrix product, as in the Ozaki
+// scheme II for emulating FP64 GEMM with int8 matrix units (Ozaki, Uchino,
+// Imamura 2025).  For p close to 2^26 and k = 8192 it needs 9 int8 products
+// where the base-256 digit form needs 16.
+#pragma once
+#include "common.h"
+
+namespace rns {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int NMOD_MAX = 12;
+// pairwise coprime moduli <= 256, largest first:
+// 2^8, 3*5*17, 11*23, 251, 13*19, 241, 239, 233, 229, 227, 223, 211
+constexpr int QLIST[NMOD_MAX] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211};
+
+// per-launch constants of the r moduli (kernel argument)
+struct Moduli {
+    int r;
+    int q[NMOD_MAX];
+    int y[NMOD_MAX];          // (M/q_j)^{-1} mod q_j
+    double w[NMOD_MAX];       // (M/q_j) mod p
+    double invq[NMOD_MAX];    // 1/q_j
+    double wm;                // M mod p
+};
+
+// balanced int8 residue of x (any int32) mod the compile-time modulus Q
+template <int Q> __device__ __forceinline__ int bal_res(int x)
+{
+    int v = x % Q;
+    v += v < 0 ? Q : 0;
+    return v >= (Q + 1) / 2 ? v - Q : v;
+}
+
+// Balanced residue of x (|x| < 2^25) mod the compile-time modulus Q in FP32:
+// with x = xh 4096 + xl (0 <= xl < 4096, |xh| <= 2^13), t = xh (4096 mod Q)
+// + xl is congruent to x and |t| < 2^21, so t, the product t (1/Q) and
+// t - f Q are exact or within one of the integer quotient; one fix-up each
+// way gives t mod Q in [0, Q).  xh and xl come in as floats.
+template <int Q> __device__ __forceinline__ int bal_res_f(float xh, float xl)
+{
+    constexpr float C = (float)(4096 % Q), QF = (float)Q, IQ = 1.0f / (float)Q;
+    const float t = fmaf(xh, C, xl);
+    float r = fmaf(-floorf(t * IQ), QF, t);
+    r += r < 0.0f ? QF : 0.0f;
+    r -= r >= QF ? QF : 0.0f;
+    r -= r >= (float)((Q + 1) / 2) ? QF : 0.0f;
+    return (int)r;
+}
+
+// balanced residue of v y mod Q for |v| <= 128, 0 <= y < 256 (|v y| < 2^15)
+template <int Q> __device__ __forceinline__ int bal_mul_f(int v, int y)
+{
+    constexpr float QF = (float)Q, IQ = 1.0f / (float)Q;
+    const float t = (float)(v * y);
+    float r = fmaf(-floorf(t * IQ), QF, t);
+    r += r < 0.0f ? QF : 0.0f;
+    r -= r >= QF ? QF : 0.0f;
+    r -= r >= (float)((Q + 1) / 2) ? QF : 0.0f;
+    return (int)r;
+}
+
+// the 16 balanced residue bytes of x[0..15] mod Q, packed little-endian;
+// mulY: each residue times y (mod Q) first (the B side)
+template <int Q, bool mulY> __device__ __forceinline__ v4i pack16q(const float *xh, const float *xl, int y)
+{
+    v4i w;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        uint32_t u = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int v = bal_res_f<Q>(xh[4 * d + e], xl[4 * d + e]);
+            if (mulY) v = bal_mul_f<Q>(v, y);
+            u |= ((uint32_t)v & 255u) << (8 * e);
+        }
+        w[d] = (int)u;
+    }
+    return w;
+}
+
+// x -> (xh, xl) as floats: x = xh 4096 + xl with 0 <= xl < 4096
+__device__ __forceinline__ void split12(int x, float &xh, float &xl)
+{
+    xh = (float)(x >> 12);
+    xl = (float)(x & 4095);
+}
+
+// the same for modulus q_j (a wave-uniform j: one branch per plane)
+template <bool mulY> __device__ __forceinline__ v4i pack16(const float *xh, const float *xl, int j, const Moduli &md)
+{
+    switch (j) {
+    case 0: return pack16q<QLIST[0], mulY>(xh, xl, md.y[0]);
+    case 1: return pack16q<QLIST[1], mulY>(xh, xl, md.y[1]);
+    case 2: return pack16q<QLIST[2], mulY>(xh, xl, md.y[2]);
+    case 3: return pack16q<QLIST[3], mulY>(xh, xl, md.y[3]);
+    case 4: return pack16q<QLIST[4], mulY>(xh, xl, md.y[4]);
+    case 5: return pack16q<QLIST[5], mulY>(xh, xl, md.y[5]);
+    case 6: return pack16q<QLIST[6], mulY>(xh, xl, md.y[6]);
+    case 7: return pack16q<QLIST[7], mulY>(xh, xl, md.y[7]);
+    case 8: return pack16q<QLIST[8], mulY>(xh, xl, md.y[8]);
+    case 9: return pack16q<QLIST[9], mulY>(xh, xl, md.y[9]);
+    case 10: return pack16q<QLIST[10], mulY>(xh, xl, md.y[10]);
+    default: return pack16q<QLIST[11], mulY>(xh, xl, md.y[11]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// core: one int8 GEMM per modulus (blockIdx.y), D = X * Y^T with X the n side
+// (planes [r][Np][Kp], row = a column of B) and Y the m side (planes
+// [r][Mp][Kp], row = a row of A); D[i][j] is C(j, i), written as the residue
+// byte e = D mod q at E[plane][i * Mp + j] (C's column-major layout, ld Mp).
+// ---------------------------------------------------------------------------
+
+// TN x TM output tile (n side x m side) per workgroup of WGN x WGM waves; BK
+// k per LDS stage (two stages, one barrier per stage)
+template <int TN_, int TM_, int WGN_, int WGM_, int BK_>
+struct Shape {
+    static constexpr int TN = TN_, TM = TM_, WGN = WGN_, WGM = WGM_, BK = BK_;
+    static constexpr int NT = 64 * WGN * WGM;
+    static constexpr int WTN = TN / WGN, WTM = TM / WGM;  // wave tile
+    static constexpr int FN = WTN / 32, FM = WTM / 32;    // 32x32 blocks per wave
+    static constexpr int CPR = BK / 16;                   // 16-byte chunks per LDS row
+    static constexpr int KK = BK / 32;                    // MFMA k-steps per stage
+};
+
+// LDS rows of BK bytes hold CPR 16-byte chunks; chunk q of row `row` sits in
+// slot q ^ f(row) so that 8 consecutive rows read at one chunk (the 8 lanes
+// of one LDS cycle of a fragment read) and one row written by 8 lanes both
+// touch 8 distinct 16-byte bank groups
+template <int CPR> __device__ __forceinline__ int slot(int row, int q)
+{
+    if constexpr (CPR == 8) return q ^ (row & 7);
+    else if constexpr (CPR == 4) return q ^ ((row >> 1) & 3);
+    else return q ^ ((row >> 2) & 1);
+}
+
+template <class SH>
+__global__ __launch_bounds__(SH::NT, 1) void k_rns_gemm(const int8_t *__restrict__ X, const int8_t *__restrict__ Y,
+                                                      int64_t Np, int64_t Mp, int64_t Kp,
+                                                      uint8_t *__restrict__ E, Moduli md)
+{
+    constexpr int TN = SH::TN, TM = SH::TM, BK = SH::BK, NT = SH::NT, CPR = SH::CPR, KK = SH::KK;
+    constexpr int FN = SH::FN, FM = SH::FM;
+    constexpr int XC = TN * CPR / NT, YC = TM * CPR / NT;  // 16-byte chunks per thread per stage
+    static_assert(TN * CPR % NT == 0 && TM * CPR % NT == 0, "stage chunks must divide among the threads");
+    __shared__ __attribute__((aligned(16))) int8_t Xs[2][TN * BK];
+    __shared__ __attribute__((aligned(16))) int8_t Ys[2][TM * BK];
+
+    int64_t ti, tj;
+    if (!tile_of_block(Np / TN, Mp / TM, &ti, &tj)) return;
+    const int plane = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wn = wave % SH::WGN, wm = wave / SH::WGN;
+    const int64_t n0 = ti * TN, m0 = tj * TM;
+    const int8_t *Xg = X + (size_t)plane * Np * Kp + (size_t)n0 * Kp;
+    const int8_t *Yg = Y + (size_t)plane * Mp * Kp + (size_t)m0 * Kp;
+
+    v16i acc[FN][FM];
+#pragma unroll
+    for (int a = 0; a < FN; ++a)
+#pragma unroll
+        for (int b = 0; b < FM; ++b) acc[a][b] = v16i{0};
+
+    v4i rx[XC], ry[YC];
+    auto gload = [&](int64_t k0) {
+#pragma unroll
+        for (int c = 0; c < XC; ++c) {
+            const int idx = tid + NT * c, row = idx / CPR, q = idx % CPR;
+            rx[c] = *reinterpret_cast<const v4i *>(Xg + (size_t)row * Kp + k0 + 16 * q);
+        }
+#pragma unroll
+        for (int c = 0; c < YC; ++c) {
+            const int idx = tid + NT * c, row = idx / CPR, q = idx % CPR;
+            ry[c] = *reinterpret_cast<const v4i *>(Yg + (size_t)row * Kp + k0 + 16 * q);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int c = 0; c < XC; ++c) {
+            const int idx = tid + NT * c, row = idx / CPR, q = idx % CPR;
+            *reinterpret_cast<v4i *>(&Xs[buf][row * BK + 16 * slot<CPR>(row, q)]) = rx[c];
+        }
+#pragma unroll
+        for (int c = 0; c < YC; ++c) {
+            const int idx = tid + NT * c, row = idx / CPR, q = idx % CPR;
+            *reinterpret_cast<v4i *>(&Ys[buf][row * BK + 16 * slot<CPR>(row, q)]) = ry[c];
+        }
+    };
+    // fragments of k-step kk: lane holds row (lane % 32) of a 32-row block and
+    // the 16 bytes of chunk 2 kk + lane / 32
+    v4i xf[2][FN], yf[2][FM];
+    auto lfrag = [&](int buf, int kk, int set) {
+        const int q = 2 * kk + (lane >> 5);
+#pragma unroll
+        for (int a = 0; a < FN; ++a) {
+            const int row = (wn * FN + a) * 32 + (lane & 31);
+            xf[set][a] = *reinterpret_cast<const v4i *>(&Xs[buf][row * BK + 16 * slot<CPR>(row, q)]);
+        }
+#pragma unroll
+        for (int b = 0; b < FM; ++b) {
+            const int row = (wm * FM + b) * 32 + (lane & 31);
+            yf[set][b] = *reinterpret_cast<const v4i *>(&Ys[buf][row * BK + 16 * slot<CPR>(row, q)]);
+        }
+    };
+
+    const int64_t ns = Kp / BK;
+    gload(0);
+    lstore(0);
+    if (ns > 1) gload(BK);
+    __syncthreads();
+    for (int64_t s = 0; s < ns; ++s) {
+        const int buf = (int)(s & 1);
+        // buffer buf ^ 1 was last read before the previous barrier
+        if (s + 1 < ns) lstore(buf ^ 1);
+        if (s + 2 < ns) gload((s + 2) * BK);
+        lfrag(buf, 0, 0);
+#pragma unroll
+        for (int kk = 0; kk < KK; ++kk) {
+            if (kk + 1 < KK) lfrag(buf, kk + 1, (kk + 1) & 1);
+#pragma unroll
+            for (int a = 0; a < FN; ++a)
+#pragma unroll
+                for (int b = 0; b < FM; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xf[kk & 1][a], yf[kk & 1][b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    // accumulator layout of a 32x32 block: lane holds column lane % 32 and
+    // rows 8 (e / 4) + 4 (lane / 32) + e % 4
+    const int q = md.q[plane];
+    const double invq = md.invq[plane];
+    uint8_t *Eg = E + (size_t)plane * Np * Mp;
+#pragma unroll
+    for (int a = 0; a < FN; ++a)
+#pragma unroll
+        for (int b = 0; b < FM; ++b) {
+            const int64_t j = m0 + (wm * FM + b) * 32 + (lane & 31);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int64_t i = n0 + (wn * FN + a) * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                const int v = acc[a][b][e];
+                int rr = v - q * (int)floor((double)v * invq);
+                rr += rr < 0 ? q : 0;
+                rr -= rr >= q ? q : 0;
+                Eg[(size_t)i * Mp + j] = (uint8_t)rr;
+            }
+        }
+}
+
+
+// The same core with an explicitly scheduled main loop.  Per thread, every
+// global chunk of a stage sits at one 32-bit offset from a workgroup-uniform
+// base (rows 32 apart add a uniform step) and every LDS chunk at one offset
+// plus an immediate; the fragments of k-step kk + 1 are read while the
+// MFMAs of k-step kk run (two register sets), the staged registers of stage
+// s + 1 are written to LDS during the first two k-steps and stage s + 2 is
+// loaded during the last two, each group interleaved with the MFMAs by
+// scheduling barriers.  Steady-state stages are branch free (the last two
+// are peeled).
+template <class SH>
+__global__ __launch_bounds__(SH::NT, 1) void k_rns_gemm2(const int8_t *__restrict__ X, const int8_t *__restrict__ Y,
+                                                       int64_t Np, int64_t Mp, int64_t Kp,
+                                                       uint8_t *__restrict__ E, Moduli md)
+{
+    constexpr int TN = SH::TN, TM = SH::TM, BK = SH::BK, NT = SH::NT, CPR = SH::CPR, KK = SH::KK;
+    constexpr int FN = SH::FN, FM = SH::FM;
+    constexpr int XC = TN * CPR / NT, YC = TM * CPR / NT;
+    constexpr int RSTEP = NT / CPR;  // rows between a thread's consecutive chunks
+    static_assert(CPR == 8 && KK == 4, "this schedule is written for 128-deep stages");
+    static_assert(XC % 2 == 0 && YC % 2 == 0, "stage chunks are written and loaded in halves");
+    __shared__ __attribute__((aligned(16))) int8_t Xs[2][TN * BK];
+    __shared__ __attribute__((aligned(16))) int8_t Ys[2][TM * BK];
+
+    int64_t ti, tj;
+    if (!tile_of_block(Np / TN, Mp / TM, &ti, &tj)) return;
+    const int plane = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wn = wave % SH::WGN, wm = wave / SH::WGN;
+    const int64_t n0 = ti * TN, m0 = tj * TM;
+    const int8_t *Xb = X + (size_t)plane * Np * Kp + (size_t)n0 * Kp;
+    const int8_t *Yb = Y + (size_t)plane * Mp * Kp + (size_t)m0 * Kp;
+    const int64_t rstep = (int64_t)RSTEP * Kp;
+
+    // staging: chunk c of this thread is row tid / CPR + RSTEP c, chunk tid % CPR
+    const int srow = tid / CPR, sq = tid % CPR;
+    const uint32_t goff = (uint32_t)(srow * Kp + 16 * sq);
+    const uint32_t loff = (uint32_t)(srow * BK + 16 * slot<CPR>(srow, sq));
+    // fragments: rows (w * F + a) * 32 + lane % 32 all share row % 8 = lane % 8
+    const int h = lane >> 5, l7 = lane & 7;
+    uint32_t fx[KK], fy[KK];
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk) {
+        const int sl = (2 * kk + h) ^ l7;
+        fx[kk] = (uint32_t)((wn * FN * 32 + (lane & 31)) * BK + 16 * sl);
+        fy[kk] = (uint32_t)((wm * FM * 32 + (lane & 31)) * BK + 16 * sl);
+    }
+
+    v16i acc[FN][FM];
+#pragma unroll
+    for (int a = 0; a < FN; ++a)
+#pragma unroll
+        for (int b = 0; b < FM; ++b) acc[a][b] = v16i{0};
+
+    v4i rx[XC], ry[YC];
+    v4i xf[2][FN], yf[2][FM];
+
+    auto gload_half = [&](int64_t k0, int half) {
+#pragma unroll
+        for (int c = half * XC / 2; c < (half + 1) * XC / 2; ++c)
+            rx[c] = *reinterpret_cast<const v4i *>(Xb + c * rstep + k0 + goff);
+#pragma unroll
+        for (int c = half * YC / 2; c < (half + 1) * YC / 2; ++c)
+            ry[c] = *reinterpret_cast<const v4i *>(Yb + c * rstep + k0 + goff);
+    };
+    auto lstore_half = [&](int buf, int half) {
+#pragma unroll
+        for (int c = half * XC / 2; c < (half + 1) * XC / 2; ++c)
+            *reinterpret_cast<v4i *>(&Xs[buf][loff + c * RSTEP * BK]) = rx[c];
+#pragma unroll
+        for (int c = half * YC / 2; c < (half + 1) * YC / 2; ++c)
+            *reinterpret_cast<v4i *>(&Ys[buf][loff + c * RSTEP * BK]) = ry[c];
+    };
+    auto lfrag = [&](int buf, int kk, int set) {
+#pragma unroll
+        for (int a = 0; a < FN; ++a) xf[set][a] = *reinterpret_cast<const v4i *>(&Xs[buf][fx[kk] + a * 32 * BK]);
+#pragma unroll
+        for (int b = 0; b < FM; ++b) yf[set][b] = *reinterpret_cast<const v4i *>(&Ys[buf][fy[kk] + b * 32 * BK]);
+    };
+    auto mfmas = [&](int set) {
+#pragma unroll
+        for (int a = 0; a < FN; ++a)
+#pragma unroll
+            for (int b = 0; b < FM; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xf[set][a], yf[set][b], acc[a][b], 0, 0, 0);
+    };
+    constexpr int NMF = FN * FM, NFR = FN + FM, NST = (XC + YC) / 2, NGL = (XC + YC) / 2;
+    // one k-step: its MFMAs interleaved with the fragment reads of the next
+    // k-step and (when asked) half of the stage's LDS writes or global loads
+    auto step = [&](int buf, int kk, bool st_half, int sthalf, int nbuf, bool gl_half, int glhalf, int64_t k2) {
+        const int set = kk & 1;
+        if (kk + 1 < KK) lfrag(buf, kk + 1, set ^ 1);
+        if (st_half) lstore_half(nbuf, sthalf);
+        if (gl_half) gload_half(k2, glhalf);
+        mfmas(set);
+        // the pattern: an MFMA, then one memory instruction, repeated
+#pragma unroll
+        for (int t = 0; t < NMF; ++t) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            if (kk + 1 < KK && t < NFR) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            if (st_half && t < NST) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+            if (gl_half && t < NGL) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+        }
+    };
+
+    const int64_t ns = Kp / BK;
+    gload_half(0, 0);
+    gload_half(0, 1);
+    lstore_half(0, 0);
+    lstore_half(0, 1);
+    if (ns > 1) {
+        gload_half(BK, 0);
+        gload_half(BK, 1);
+    }
+    __syncthreads();
+    lfrag(0, 0, 0);
+    int64_t s = 0;
+    // steady state: stage s stores stage s + 1 and loads stage s + 2
+    for (; s + 2 < ns; ++s) {
+        const int buf = (int)(s & 1);
+        const int64_t k2 = (s + 2) * BK;
+        step(buf, 0, true, 0, buf ^ 1, false, 0, 0);
+        step(buf, 1, true, 1, buf ^ 1, false, 0, 0);
+        step(buf, 2, false, 0, 0, true, 0, k2);
+        step(buf, 3, false, 0, 0, true, 1, k2);
+        __syncthreads();
+        lfrag(buf ^ 1, 0, 0);
+    }
+    if (s + 1 < ns) {
+        const int buf = (int)(s & 1);
+        step(buf, 0, true, 0, buf ^ 1, false, 0, 0);
+        step(buf, 1, true, 1, buf ^ 1, false, 0, 0);
+        step(buf, 2, false, 0, 0, false, 0, 0);
+        step(buf, 3, false, 0, 0, false, 0, 0);
+        __syncthreads();
+        lfrag(buf ^ 1, 0, 0);
+        ++s;
+    }
+    {
+        const int buf = (int)(s & 1);
+#pragma unroll
+        for (int kk = 0; kk < KK; ++kk) step(buf, kk, false, 0, 0, false, 0, 0);
+    }
+
+    const int q = md.q[plane];
+    const double invq = md.invq[plane];
+    uint8_t *Eg = E + (size_t)plane * Np * Mp;
+#pragma unroll
+    for (int a = 0; a < FN; ++a)
+#pragma unroll
+        for (int b = 0; b < FM; ++b) {
+            const int64_t j = m0 + (wm * FM + b) * 32 + (lane & 31);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int64_t i = n0 + (wn * FN + a) * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                const int v = acc[a][b][e];
+                int rr = v - q * (int)floor((double)v * invq);
+                rr += rr < 0 ? q : 0;
+                rr -= rr >= q ? q : 0;
+                Eg[(size_t)i * Mp + j] = (uint8_t)rr;
+            }
+        }
+}
+
+}  // namespace rns
+
+namespace rns {
+
+// ---------------------------------------------------------------------------
+// split: operand residues -> balanced int8 residue planes
+// ---------------------------------------------------------------------------
+
+// balanced representative of a storage element, |x| <= floor(p/2)
+template <typename T> __device__ __forceinline__ int balanced(T v, const ModP &mp)
+{
+    const int x = (int)to_res<T>(v, mp);
+    return x > mp.half ? x - (int)mp.p32 : x;
+}
+
+// A (m x k column-major) -> planes Y[j][row][kk] (k contiguous), zero-padded
+// to Mp x Kp; 64x64 tiles transposed through LDS.  blockIdx.x: k tile,
+// blockIdx.y: row tile.
+template <typename TA>
+__global__ __launch_bounds__(256) void k_split_rows(int64_t m, int64_t k, const TA *__restrict__ A, int64_t lda,
+                                                   int8_t *__restrict__ Yp, int64_t Mp, int64_t Kp, ModP mp,
+                                                   Moduli md, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    __shared__ int32_t tile[64][65];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 64;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = tid & 63, kk = (tid >> 6) + 4 * i;
+        const int64_t gr = r0 + row, gk = k0 + kk;
+        tile[row][kk] = (gr < m && gk < k) ? balanced<TA>(A[gr + gk * lda], mp) : 0;
+    }
+    __syncthreads();
+    const int row = tid >> 2, ch = tid & 3;
+    float xh[16], xl[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) split12(tile[row][ch * 16 + e], xh[e], xl[e]);
+    const size_t plane = (size_t)Mp * Kp;
+    int8_t *dst = Yp + (size_t)(r0 + row) * Kp + k0 + ch * 16;
+#pragma unroll 1
+    for (int j = 0; j < md.r; ++j) *reinterpret_cast<v4i *>(dst + j * plane) = pack16<false>(xh, xl, j, md);
+}
+
+// B (k x n column-major) -> planes X[j][col][kk] (k contiguous, times y_j),
+// zero-padded to Np x Kp.  Same orientation: each thread maps 16 consecutive
+// k of one column.
+template <typename TB>
+__global__ __launch_bounds__(256) void k_split_cols(int64_t k, int64_t n, const TB *__restrict__ B, int64_t ldb,
+                                                   int8_t *__restrict__ Xp, int64_t Np, int64_t Kp, ModP mp,
+                                                   Moduli md, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int64_t chunks = Kp / 16;
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= Np * chunks) return;
+    const int64_t col = e / chunks, k0 = (e % chunks) * 16;
+    float xh[16], xl[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const int64_t gk = k0 + t;
+        split12((col < n && gk < k) ? balanced<TB>(B[gk + col * ldb], mp) : 0, xh[t], xl[t]);
+    }
+    const size_t plane = (size_t)Np * Kp;
+    int8_t *dst = Xp + (size_t)col * Kp + k0;
+#pragma unroll 1
+    for (int j = 0; j < md.r; ++j) *reinterpret_cast<v4i *>(dst + j * plane) = pack16<true>(xh, xl, j, md);
+}
+
+// ---------------------------------------------------------------------------
+// combine: the r residue bytes of each entry -> C = epi(C, C_int mod p)
+// ---------------------------------------------------------------------------
+template <typename TC, int EPI>
+__global__ __launch_bounds__(256) void k_combine(int64_t m, int64_t n, const uint8_t *__restrict__ E, int64_t Mp,
+                                                int64_t plane, Moduli md, TC *__restrict__ C, int64_t ldc,
+                                                ModP mp, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    // four consecutive rows of one column per thread: one 4-byte load per
+    // modulus (Mp is a multiple of the output tile, so the load is aligned
+    // and stays inside the plane)
+    const int64_t mq = (m + 3) / 4;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= mq * n) return;
+    const int64_t j0 = 4 * (idx % mq), i = idx / mq;
+    const uint8_t *src = E + i * Mp + j0;
+    double S[4] = {0.0, 0.0, 0.0, 0.0}, T[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int jj = 0; jj < NMOD_MAX; ++jj) {
+        if (jj < md.r) {
+            const uint32_t w4 = *reinterpret_cast<const uint32_t *>(src + (size_t)jj * plane);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double v = (double)((w4 >> (8 * e)) & 255u);
+                S[e] = fma(v, md.w[jj], S[e]);  // < 12 * 2^8 * 2^26: exact
+                T[e] = fma(v, md.invq[jj], T[e]);
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        if (j0 + e < m) {
+            const double t = rint(T[e]);
+            uint32_t r = (uint32_t)dmod(fma(-t, md.wm, S[e]), mp);
+            TC *dst = C + j0 + e + i * ldc;
+            if (EPI == EPI_SUB) r = submod(to_res<TC>(*dst, mp), r, mp);
+            else if (EPI == EPI_ADD) r = addmod(to_res<TC>(*dst, mp), r, mp);
+            *dst = from_res<TC>(r);
+        }
+    }
+}
+
+}  // namespace rns

@@ -61,8 +61,8 @@ int time_config(modp_ctx *ctx, const GemmArgs &g, const GemmConfig &cfg, double 
 std::vector<GemmConfig> candidates(const modp_ctx *ctx, const GemmArgs &g)
 {
     std::vector<GemmConfig> out;
-    const int forms[2] = {MODP_FORM_PLAIN_F64, MODP_FORM_LIMB_I8};
-    const int tiles[3][2] = {{64, 64}, {128, 64}, {128, 128}};
+    const int forms[3] = {MODP_FORM_PLAIN_F64, MODP_FORM_LIMB_I8, MODP_FORM_RNS_I8};
+    const int tiles[4][2] = {{64, 64}, {128, 64}, {128, 128}, {256, 256}};
     const int splits[4] = {1, 2, 4, 8};
     for (int f : forms)
         for (const auto &t : tiles)
@@ -181,8 +181,10 @@ int tune_import(modp_ctx *ctx, const char *text)
             if (sscanf(line.c_str(), "g %d %d %d %d %d %d %d %d %d %d %d %lf", &k.v[0], &k.v[1], &k.v[2], &k.v[3],
                        &k.v[4], &k.v[5], &k.v[6], &e.cfg.form, &e.cfg.tile_m, &e.cfg.tile_n, &e.cfg.ksplit,
                        &e.ms) != 12 ||
-                k.v[0] < 0 || k.v[0] > 2 || (e.cfg.form != MODP_FORM_PLAIN_F64 && e.cfg.form != MODP_FORM_LIMB_I8) ||
-                (e.cfg.tile_m != 64 && e.cfg.tile_m != 128) || (e.cfg.tile_n != 64 && e.cfg.tile_n != 128) ||
+                k.v[0] < 0 || k.v[0] > 2 ||
+                (e.cfg.form != MODP_FORM_PLAIN_F64 && e.cfg.form != MODP_FORM_LIMB_I8 && e.cfg.form != MODP_FORM_RNS_I8) ||
+                (e.cfg.tile_m != 64 && e.cfg.tile_m != 128 && e.cfg.tile_m != 256) ||
+                (e.cfg.tile_n != 64 && e.cfg.tile_n != 128 && e.cfg.tile_n != 256) ||
                 e.cfg.ksplit < 1 || e.cfg.ksplit > 16)
                 return set_error(ctx, MODP_ERR_ARG, "autotune cache line %d malformed", lineno);
         } else if (line[0] == 'b') {

@@ -28,7 +28,7 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _lib
-from ._lib import FORM_AUTO, FORM_LIMB_I8, FORM_PLAIN_F64, GemmJob, ElimJob
+from ._lib import FORM_AUTO, FORM_LIMB_I8, FORM_PLAIN_F64, FORM_RNS_I8, GemmJob, ElimJob
 from ._lib import TUNE_CACHED, TUNE_OFF, TUNE_ON_MISS
 
 TILE_WIDTH = 32
@@ -42,7 +42,7 @@ __all__ = [
     "perm_array_to_matrix", "gemm_batched", "GemmBatchPlan", "elim_batched",
     "CuModArraySizeMismatchException", "CuModArrayModulusMismatchException",
     "CuModMatrixNotSquareException", "CuModMatrixModulusNotPrimeException", "ArgumentError",
-    "DimensionMismatch", "FORM_AUTO", "FORM_PLAIN_F64", "FORM_LIMB_I8", "context",
+    "DimensionMismatch", "FORM_AUTO", "FORM_PLAIN_F64", "FORM_LIMB_I8", "FORM_RNS_I8", "context",
 ]
 
 
@@ -435,6 +435,14 @@ def change_modulus(A: CuModArray, new_N: int) -> CuModArray:
     return R
 
 
+def change_modulus_no_alloc_(A: CuModArray, new_N: int) -> CuModArray:
+    """change_modulus_no_alloc!(A, new_N) (CuModMatrix.jl:751-758): A's
+    storage reduced in place when new_N < A.N, returned under modulus new_N."""
+    if new_N < A.N:
+        mod_(A, A, new_N)
+    return CuModArray(A.data, A.padded, A.size, new_N, A.elem_type)
+
+
 def copy(A: CuModArray) -> CuModArray:
     """CuModcopy (CuModMatrix.jl:838-840)."""
     R = _new(A.size[0], A.size[1], A.N, A.elem_type, A.ctx, zero=False, padded=A.padded)
@@ -613,6 +621,50 @@ def crt(A1: CuModArray, A2: CuModArray) -> CuModArray:
     ctx = A1.ctx
     ctx.check(ctx.lib.modp_crt2(ctx.h, A1.dtype_code, m, n, A1.ptr, A1.ld, A1.N, A2.ptr, A2.ld, A2.N, Y.ptr, Y.ld),
               "crt")
+    return Y
+
+
+def crt_words(moduli: Sequence[int], signed: bool = False) -> int:
+    """32-bit words needed for the product of the moduli (and a sign bit)."""
+    import math
+    bits = sum(math.log2(int(q)) for q in moduli) + (1 if signed else 0)
+    return int(math.ceil(bits / 32 - 1e-12))
+
+
+def crt_reconstruct(mats: Sequence[CuModArray], signed: bool = False) -> np.ndarray:
+    """Chinese remaindering of residue matrices of equal size under pairwise
+    coprime moduli (their N) into the integer matrix y with y = A_i mod N_i:
+    y in [0, P) or, signed, in (-P/2, P/2], P = prod N_i.  Returns a numpy
+    object array of Python integers (the multi-word device result
+    reassembled on the host)."""
+    if not mats:
+        raise ArgumentError("CRT needs at least one residue matrix")
+    m, n = mats[0].size
+    et = mats[0].elem_type
+    for A in mats:
+        if A.size != (m, n):
+            raise CuModArraySizeMismatchException("CRT needs matrices of the same size")
+        if A.elem_type != et:
+            raise ArgumentError("Mismatched element types")
+    ctx = mats[0].ctx
+    r = len(mats)
+    moduli = [int(A.N) for A in mats]
+    words = crt_words(moduli, signed)
+    out = np.zeros(max(1, words * m * n), dtype=np.uint32)
+    buf = ctx.malloc(4 * out.size)
+    ptrs = (ctypes.c_void_p * r)(*[A.ptr for A in mats])
+    lds = (ctypes.c_int64 * r)(*[A.ld for A in mats])
+    ps = (ctypes.c_int64 * r)(*moduli)
+    ctx.check(ctx.lib.modp_crt(ctx.h, mats[0].dtype_code, m, n, r, ptrs, lds, ps, 1 if signed else 0, buf.ptr,
+                               max(m, 1), words), "crt")
+    buf.download(out)
+    planes = out[:words * m * n].reshape(words, n, m)  # plane w: column-major m x n
+    Y = np.zeros((m, n), dtype=object)
+    for w in range(words - 1, -1, -1):
+        Y = Y * (1 << 32) + planes[w].T.astype(object)
+    if signed:
+        top = 1 << (32 * words)
+        Y = np.where(Y >= top // 2, Y - top, Y)
     return Y
 
 
@@ -801,6 +853,23 @@ def gemm_batched(jobs, elem_type=np.float64):
     plan = GemmBatchPlan(jobs, elem_type)
     plan.run()
     plan.ctx.synchronize()
+
+
+class ElimBatchPlan:
+    """A captured batch of independent eliminations (ElimJob descriptors over
+    device memory); run() is asynchronous on the context's stream."""
+
+    def __init__(self, op: int, jobs_raw: Sequence[ElimJob], elem_type=np.float64, ctx: Optional[Context] = None):
+        self.ctx = ctx or context()
+        arr = (ElimJob * max(1, len(jobs_raw)))(*jobs_raw)
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.modp_elim_batched_plan(self.ctx.h, int(op), _dtype_code(elem_type), arr,
+                                                           len(jobs_raw), ctypes.byref(h)), "elim_batched_plan")
+        self.h = h
+        self._fin = weakref.finalize(self, self.ctx.lib.modp_elim_batched_destroy, self.ctx.h, h)
+
+    def run(self):
+        self.ctx.check(self.ctx.lib.modp_elim_batched_run(self.ctx.h, self.h), "elim_batched_run")
 
 
 def elim_batched(op: int, jobs_raw: Sequence[ElimJob], elem_type=np.float64, ctx: Optional[Context] = None):

@@ -35,6 +35,7 @@ MODP_F32 = 1
 FORM_AUTO = 0
 FORM_PLAIN_F64 = 1
 FORM_LIMB_I8 = 2
+FORM_RNS_I8 = 3
 
 TUNE_OFF = 0
 TUNE_CACHED = 1
@@ -98,6 +99,7 @@ SIGNATURES = [
     ("modp_lu", _int, [_vp, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i64,
                        _p64, _p64, _p64]),
     ("modp_rref", _int, [_vp, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _i64, _p64, _p64]),
+    ("modp_crt", _int, [_vp, _int, _i64, _i64, _int, ctypes.POINTER(_vp), _p64, _p64, _int, _vp, _i64, _i64]),
     ("modp_crt2", _int, [_vp, _int, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64]),
     ("modp_trtri", _int, [_vp, _int, _int, _i64, _i64, _vp, _i64, _vp, _i64, _i64]),
     ("modp_inverse", _int, [_vp, _int, _int, _i64, _vp, _i64, _vp, _i64, _i64, ctypes.POINTER(_int)]),
@@ -108,6 +110,9 @@ SIGNATURES = [
     ("modp_gemm_batched_destroy", _int, [_vp, _vp]),
     ("modp_gemm_batched", _int, [_vp, _int, ctypes.POINTER(GemmJob), _i64]),
     ("modp_elim_batched", _int, [_vp, _int, _int, ctypes.POINTER(ElimJob), _i64]),
+    ("modp_elim_batched_plan", _int, [_vp, _int, _int, ctypes.POINTER(ElimJob), _i64, ctypes.POINTER(_vp)]),
+    ("modp_elim_batched_run", _int, [_vp, _vp]),
+    ("modp_elim_batched_destroy", _int, [_vp, _vp]),
     ("modp_kernel_timing", _int, [_vp, _int]),
     ("modp_kernel_time", _int, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), _p64]),
     ("modp_kernel_timing_reset", _int, [_vp]),

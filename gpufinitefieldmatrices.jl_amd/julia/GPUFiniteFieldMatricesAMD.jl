@@ -17,7 +17,7 @@ export ModMatrix, CuModMatrix, pluq_gpu_kernel, pluq, lu_ple, lu_gpu_type, rref,
        stripe_mul!, find_max_stripe_ops, mat_mul_gpu_type, mat_mul_type_inplace!,
        inverse, is_invertible, is_invertible_with_inverse,
        upper_triangular_inverse_no_copy, lower_triangular_inverse_no_copy, crt,
-       eye, mod_elements!, change_modulus, change_modulus_no_alloc!
+       eye, mod_elements!, change_modulus, change_modulus_no_alloc!, crt_reconstruct
 
 const LIB = get(ENV, "MODP_HIP_LIB", joinpath(@__DIR__, "..", "gpufinitefieldmatrices", "libmodp_hip.so"))
 const TILE_WIDTH = 32
@@ -44,7 +44,7 @@ struct MatrixNotInvertibleException <: Exception
 end
 
 const MODP_OK = 0
-const FORM_AUTO, FORM_PLAIN_F64, FORM_LIMB_I8 = 0, 1, 2
+const FORM_AUTO, FORM_PLAIN_F64, FORM_LIMB_I8, FORM_RNS_I8 = 0, 1, 2, 3
 
 # ---- context ----------------------------------------------------------------
 mutable struct Context
@@ -428,6 +428,44 @@ function crt(A1::ModMatrix{T}, A2::ModMatrix{T}) where {T}
                 (Ptr{Cvoid}, Cint, Int64, Int64, Ptr{Cvoid}, Int64, Int64, Ptr{Cvoid}, Int64, Int64, Ptr{Cvoid}, Int64),
                 context().h, dtype_code(T), rows(A1), cols(A1), A1.data.ptr, ld(A1), A1.N, A2.data.ptr, ld(A2), A2.N,
                 Y.data.ptr, ld(Y)), "crt")
+    return Y
+end
+
+"""
+    crt_reconstruct(mats; signed=false) -> Matrix{BigInt}
+
+Chinese remaindering of residue matrices of equal size under pairwise
+coprime moduli (their N, at most 12 of them): y with y == mats[i] mod N_i,
+in [0, P) or, signed, in (-P/2, P/2], P = prod N_i.  The device writes the
+little-endian 32-bit words of every entry (`modp_crt`); they are assembled
+into BigInts here.
+"""
+function crt_reconstruct(mats::Vector{ModMatrix{T}}; signed::Bool=false) where {T}
+    isempty(mats) && throw(ArgumentError("CRT needs at least one residue matrix"))
+    m, n = size(mats[1])
+    all(A -> size(A) == (m, n), mats) || throw(CuModArraySizeMismatchException("CRT needs matrices of the same size"))
+    moduli = Int64[A.N for A in mats]
+    words = ceil(Int, (sum(log2.(Float64.(moduli))) + (signed ? 1 : 0)) / 32 - 1e-12)
+    buf = DeviceBuffer(4 * max(1, words * m * n))
+    ptrs = Ptr{Cvoid}[A.data.ptr for A in mats]
+    lds = Int64[ld(A) for A in mats]
+    check(ccall((:modp_crt, LIB), Cint,
+                (Ptr{Cvoid}, Cint, Int64, Int64, Cint, Ptr{Ptr{Cvoid}}, Ptr{Int64}, Ptr{Int64}, Cint, Ptr{Cvoid},
+                 Int64, Int64),
+                context().h, dtype_code(T), m, n, length(mats), ptrs, lds, moduli, signed ? 1 : 0, buf.ptr,
+                max(m, 1), words), "crt")
+    host = Vector{UInt32}(undef, max(1, words * m * n))
+    check(ccall((:modp_memcpy_d2h, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Csize_t),
+                context().h, host, buf.ptr, sizeof(host)), "download")
+    Y = Matrix{BigInt}(undef, m, n)
+    top = BigInt(1) << (32 * words)
+    for j in 1:n, i in 1:m
+        y = BigInt(0)
+        for w in words:-1:1
+            y = (y << 32) + host[(w - 1) * m * n + (j - 1) * m + i]
+        end
+        Y[i, j] = (signed && y >= top >> 1) ? y - top : y
+    end
     return Y
 end
 

@@ -49,8 +49,12 @@ typedef enum { MODP_F64 = 0, MODP_F32 = 1 } modp_dtype;
 typedef enum {
     MODP_FORM_AUTO = 0,      /* autotuner's choice for (shape, bits of p) */
     MODP_FORM_PLAIN_F64 = 1, /* FP64 MFMA, balanced residues, periodic reduction */
-    MODP_FORM_LIMB_I8 = 2    /* signed base-256 digits (limbs) in int8,
+    MODP_FORM_LIMB_I8 = 2,   /* signed base-256 digits (limbs) in int8,
                                 int32-accumulating MFMA, FP64 Barrett recombination */
+    MODP_FORM_RNS_I8 = 3     /* multi-modular limbs: residues modulo up to 12 pairwise
+                                coprime moduli <= 256 in int8, one int32-accumulating
+                                MFMA product per modulus, Chinese-remainder
+                                recombination in FP64 (GEMM only) */
 } modp_form;
 
 /* autotuner modes (modp_autotune_mode) */
@@ -142,10 +146,11 @@ int modp_gemv(modp_ctx *ctx, int dtype, int64_t m, int64_t k, const void *A, int
  * take their configuration from the context's tuning cache when the mode
  * allows, else from built-in heuristics. */
 typedef struct {
-    int32_t form;    /* MODP_FORM_PLAIN_F64 or MODP_FORM_LIMB_I8 */
-    int32_t tile_m;  /* output tile: 64x64 or 128x128; the limb form also 128x64 */
+    int32_t form;    /* MODP_FORM_PLAIN_F64, MODP_FORM_LIMB_I8 or MODP_FORM_RNS_I8 */
+    int32_t tile_m;  /* output tile: 64x64 or 128x128; the limb form also 128x64;
+                        the RNS form 128x128 or 256x256 */
     int32_t tile_n;
-    int32_t ksplit;  /* 1..16 */
+    int32_t ksplit;  /* 1..16 (the RNS form: 1; its grid has one layer per modulus) */
 } modp_gemm_config;
 
 int modp_autotune_mode(modp_ctx *ctx, int mode);
@@ -227,6 +232,20 @@ int modp_apply_col_perm(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, vo
 int modp_crt2(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, const void *X1, int64_t ld1, int64_t p1,
               const void *X2, int64_t ld2, int64_t p2, void *Y, int64_t ldy);
 
+/* Chinese remaindering of r residue matrices (1 <= r <= 12, pairwise coprime
+ * moduli p[i] in 2..2^26; X[i] a device matrix of residues mod p[i] with
+ * leading dimension ldx[i], host arrays X, ldx, p) into integers, by
+ * Garner's mixed-radix conversion.  Y holds `words` planes of rows x cols
+ * uint32 words (leading dimension ldy, plane w at Y + w*ldy*cols), the
+ * little-endian words of y in [0, P), P = prod p[i] (sign = 0), or of the
+ * representative in (-P/2, P/2] in two's complement (sign = 1).  `words`
+ * must cover log2(P) (+1 when signed) bits and be at most 10.  Turns the
+ * per-prime results of the batched forms into one integer matrix; the
+ * reference's nearest counterpart is the two-limb composite modulus of
+ * KaratsubaMatrix (src/KaratsubaMatrix/KaratsubaMatrix.jl:18-51). */
+int modp_crt(modp_ctx *ctx, int dtype, int64_t rows, int64_t cols, int r, const void *const *X,
+             const int64_t *ldx, const int64_t *p, int sign, uint32_t *Y, int64_t ldy, int64_t words);
+
 /* ---- inverses ------------------------------------------------------------ */
 
 /* Inverse of a triangular matrix mod p.  uplo 0: A (m x n) is upper
@@ -288,10 +307,24 @@ typedef struct {
     int64_t *counts;             /* device, 3 entries: n_row, n_col, rank */
 } modp_elim_job;
 
-/* Independent eliminations of small matrices (m, n <= 128), each under its
- * own prime, one workgroup per instance with the matrix resident in LDS,
- * pulled from a device work queue by a persistent kernel.  Outputs are
- * written to device memory; the call returns without synchronising. */
+typedef struct modp_elim_plan modp_elim_plan;
+
+/* Build a plan for `count` independent eliminations of small matrices
+ * (m, n <= 128), each under its own prime (device pointers in the jobs):
+ * the jobs are sorted by cost and their table is copied to the device once;
+ * a persistent kernel, one wavefront per instance with the matrix resident in
+ * LDS, pulls instances from a device work queue, and [queue reset, kernel] is
+ * captured in a hipGraph.  Same pivot rule and outputs as modp_pluq /
+ * modp_lu / modp_rref. */
+int modp_elim_batched_plan(modp_ctx *ctx, int op, int dtype, const modp_elim_job *jobs, int64_t count,
+                           modp_elim_plan **out);
+/* Launch the plan's graph on the context's stream and return without
+ * synchronising; the outputs are ready after modp_synchronize.  Runs of one
+ * plan on one stream are ordered. */
+int modp_elim_batched_run(modp_ctx *ctx, modp_elim_plan *plan);
+/* Waits for the context's stream, then frees the plan. */
+int modp_elim_batched_destroy(modp_ctx *ctx, modp_elim_plan *plan);
+/* plan + run + wait + destroy: returns after the outputs are written. */
 int modp_elim_batched(modp_ctx *ctx, int op, int dtype, const modp_elim_job *jobs, int64_t count);
 
 /* ---- timing (for the benchmark: HIP events on the context's stream) ---- */

@@ -260,3 +260,59 @@ def test_full_elim_batch_vs_oracle(gffm, full_elim_batch, op):
         ctx.synchronize()
         bad = _elim_mismatches(op, arena.download(), full_elim_batch)
         assert bad == [], f"run {rep}: {len(bad)} instances differ, first {bad[:5]}"
+
+
+def test_elim_plan_replays_vs_oracle(gffm):
+    # a captured elimination plan replayed three times with eager work
+    # between the replays (outputs cleared, a product run); every replay
+    # equals the oracle
+    g = gffm
+    ctx = g.context()
+    inst = W.elim_instances(400, seed=11)
+    for op, code in (("pluq", 0), ("lu", 1), ("rref", 2)):
+        arena = W.ElimArena(ctx, inst)
+        plan = g.ElimBatchPlan(code, list(arena.jobs), ctx=ctx)
+        for rep in range(3):
+            arena.clear_outputs()
+            X = g.rand(np.float64, 64, 64, 251, seed=rep)
+            _ = X * X
+            plan.run()
+            ctx.synchronize()
+            bad = _elim_mismatches(op, arena.download(), inst)
+            assert bad == [], f"{op} replay {rep}: {len(bad)} instances differ, first {bad[:5]}"
+
+
+@pytest.mark.parametrize("op", ["pluq", "lu", "rref"])
+def test_elim_batch_large_primes_and_edges(gffm, op):
+    # moduli above 2^16 (32-bit residues in LDS, 128 x 128 slots above
+    # 64 KiB) mixed with small ones, and edge shapes
+    g = gffm
+    ctx = g.context()
+    rng = np.random.default_rng(12)
+    primes = [2, 65521, 65537, 8388593, 2 ** 26 - 5]
+    shapes = [(1, 1), (1, 9), (9, 1), (128, 128), (128, 3), (3, 128), (127, 128), (64, 65), (65, 64)]
+    inst = []
+    for i in range(60):
+        m, n = shapes[i % len(shapes)] if i < 2 * len(shapes) else (int(x) for x in rng.integers(1, 129, size=2))
+        p = primes[i % len(primes)]
+        A = rng.integers(0, p, size=(m, n))
+        if i % 3 == 0 and m > 1:
+            A[rng.choice(m, size=m // 2, replace=False), :] = 0
+        if i % 5 == 0 and n > 1:
+            A[:, rng.integers(0, n)] = 0
+        inst.append((A.astype(np.int64), p))
+    arena = W.ElimArena(ctx, inst)
+    code = {"pluq": 0, "lu": 1, "rref": 2}[op]
+    g.elim_batched(code, list(arena.jobs), ctx=ctx)
+    bad = _elim_mismatches(op, arena.download(), inst)
+    assert bad == [], f"{len(bad)} instances differ, first {bad[:5]}"
+
+
+def test_elim_batch_not_prime_reports_minus_one(gffm):
+    g = gffm
+    ctx = g.context()
+    A = np.array([[2, 1], [1, 1]], dtype=np.int64)  # pivot 2 has no inverse mod 4
+    arena = W.ElimArena(ctx, [(A, 4), (np.array([[1, 2], [3, 4]], dtype=np.int64), 5)])
+    g.elim_batched(0, list(arena.jobs), ctx=ctx)
+    res = arena.download()
+    assert int(res[0]["counts"][2]) == -1 and int(res[1]["counts"][2]) == 2

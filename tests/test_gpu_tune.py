@@ -8,17 +8,21 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-TILES = [(64, 64), (128, 64), (128, 128)]
+TILES = [(64, 64), (128, 64), (128, 128), (256, 256)]
 
 
 def dev(g, X, p, et=np.float64):
     return g.CuModMatrix(np.asarray(X), p, elem_type=et)
 
 
-def valid(form, tile, p):
+def valid(form, tile, p, ksplit=1):
     L = 1
     while p > 256 ** L:
         L += 1
+    if form == 3:
+        return tile in ((128, 128), (256, 256)) and ksplit == 1
+    if tile == (256, 256):
+        return False
     if form == 1:
         return tile != (128, 64)
     return tile != (128, 128) or L <= 2
@@ -27,9 +31,9 @@ def valid(form, tile, p):
 @pytest.mark.parametrize("p", [127, 65521, 8388593, 2 ** 26 - 5])
 @pytest.mark.parametrize("ksplit", [1, 3, 8])
 @pytest.mark.parametrize("tile", TILES, ids=str)
-@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("form", [1, 2, 3])
 def test_every_config_matches_oracle(gffm, form, tile, ksplit, p):
-    if not valid(form, tile, p):
+    if not valid(form, tile, p, ksplit):
         pytest.skip("configuration not offered for this modulus")
     g = gffm
     m, n, k = 150, 200, 2100
