@@ -334,7 +334,7 @@ struct modp_elim_plan {
 namespace {
 
 constexpr int EMAX = 128;
-constexpr int EBOOK = EMAX * (4 + 2 + 2);  // pivot values, pivot columns, row swaps
+constexpr int EBOOK = EMAX * (4 + 4 + 2 + 2);  // pivot values and inverses, pivot columns, row swaps
 
 // Every lane of the wavefront works on one instance, so LDS writes by one
 // lane are read by another in program order; the compiler must not move LDS
@@ -392,62 +392,176 @@ template <bool SMALL> __device__ __forceinline__ uint32_t mulmod_c(uint32_t a, u
     return SMALL ? row_update<true>(0u, a, b, mp) : (uint32_t)dmod((double)a * (double)b, mp);
 }
 
-// Rows i0..m-1 (but the pivot row) of the rank-1 update by the pivot row:
-// x_rc -= l_r pr_c over this lane's columns c0 (and c1 when WIDE).  Rows
-// whose multiplier is zero are skipped (a scalar branch); the multipliers of
-// eight rows are taken first, then all their loads are issued, then the
-// stores.
-template <typename S, bool SMALL, bool WIDE>
-__device__ __forceinline__ void update_rows(S *M, int pitch, int i0, int m, int row, int c0, int c1, uint32_t lq0,
-                                            uint32_t lq1, const ModP &mp)
-{
-    uint32_t k0 = M[row * pitch + c0], k1 = WIDE ? (uint32_t)M[row * pitch + c1] : 0u;
-    if (SMALL) {
-        k0 = k0 ? mp.p32 - k0 : 0u;
-        k1 = k1 ? mp.p32 - k1 : 0u;
+// ---------------------------------------------------------------------------
+// Blocked elimination of one instance.  Pivots are taken one at a time, but
+// each pivot's update is applied at once only to the columns of the current
+// panel (KB columns from the panel's first); the trailing columns receive the
+// panel's pivots together when the panel ends ("flush"):
+//   pivot rows, in order: V_j = (x_j - sum_{i<j} L_{r_j,i} V_i) inv_j
+//   other rows:           x_r - sum_j L_{r,j} V_j
+// with one modular reduction per element per panel instead of one per pivot
+// (delayed reduction: below 2^8 the sum stays under 2^20 in 32-bit integers,
+// below 2^16 under 2^36 in FP64; 32-bit residues flush after every pivot).
+// L_{r,j} is the entry of row r in pivot column j when pivot j was taken,
+// which the panel's immediate updates keep current; for rref those entries
+// are cleared after the flush (the rows above the pivot are reduced too).
+// A zero column under pluq's rule flushes first, so the rest-of-block test
+// and the column swap see current values.
+// ---------------------------------------------------------------------------
+
+template <int ACC> struct Acc;
+template <> struct Acc<0> {  // residues < 2^8, 32-bit integer sums
+    typedef uint32_t V;      // stored: p - v (mod p), added
+    static __device__ __forceinline__ V from(uint32_t v, const ModP &mp) { return v ? mp.p32 - v : 0u; }
+    typedef uint32_t S;
+    static __device__ __forceinline__ S init(uint32_t x) { return x; }
+    static __device__ __forceinline__ S madd(S a, uint32_t l, V v) { return a + __umul24(l, v); }
+    // a < 2^20: fl(a) is exact and the float quotient is within one
+    static __device__ __forceinline__ uint32_t reduce(S a, const ModP &mp)
+    {
+        const uint32_t q = (uint32_t)((float)a * mp.pinvf);
+        int32_t r = (int32_t)(a - __umul24(q, mp.p32));
+        r += r < 0 ? (int32_t)mp.p32 : 0;
+        r -= r >= (int32_t)mp.p32 ? (int32_t)mp.p32 : 0;
+        return (uint32_t)r;
     }
-    constexpr int RS = 8;
-    for (int i = i0; i < m; i += RS) {
-        uint32_t l[RS], x0[RS], x1[RS];
+};
+template <> struct Acc<1> {  // FP64 sums, exact below 2^53
+    typedef double V;        // stored: v, subtracted
+    static __device__ __forceinline__ V from(uint32_t v, const ModP &) { return (double)v; }
+    typedef double S;
+    static __device__ __forceinline__ S init(uint32_t x) { return (double)x; }
+    static __device__ __forceinline__ S madd(S a, uint32_t l, V v) { return fma(-(double)l, v, a); }
+    static __device__ __forceinline__ uint32_t reduce(S a, const ModP &mp) { return (uint32_t)dmod(a, mp); }
+};
+
+// Trailing columns pe..n-1 of the pending pivots (rows prow..prow+np-1) on
+// lanes over columns: lane q owns columns pe+q and (WIDE) pe+64+q; lanes
+// past the last column repeat it, computing and storing the same value.
+template <typename S, int KB, int ACC, bool WIDE>
+__device__ __forceinline__ void flush_cols(S *M, int pitch, int m, int n, int op, int prow, int np, int pe,
+                                           const int *pc, const uint32_t *inv, const uint32_t *lq0,
+                                           const uint32_t *lq1, const ModP &mp)
+{
+    typedef Acc<ACC> A;
+    constexpr bool SMALL = sizeof(S) < 4;
+    const int lane = threadIdx.x;
+    const int c0 = min(pe + lane, n - 1), c1 = min(pe + 64 + lane, n - 1);
+    typename A::V v0[KB], v1[KB];
+    // pivot rows, in order
 #pragma unroll
-        for (int t = 0; t < RS; ++t) {
-            const int r = i + t;
-            const uint32_t lv = (uint32_t)__builtin_amdgcn_readlane((int)(r < 64 ? lq0 : lq1), r & 63);
-            l[t] = (r < m && r != row) ? lv : 0u;
+    for (int jj = 0; jj < KB; ++jj) {
+        v0[jj] = v1[jj] = typename A::V(0);
+        if (jj < np) {
+            const int r = prow + jj;
+            typename A::S a0 = A::init(M[r * pitch + c0]), a1 = A::init(WIDE ? (uint32_t)M[r * pitch + c1] : 0u);
+#pragma unroll
+            for (int ii = 0; ii < jj; ++ii) {
+                const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)(r < 64 ? lq0[ii] : lq1[ii]), r & 63);
+                a0 = A::madd(a0, l, v0[ii]);
+                if (WIDE) a1 = A::madd(a1, l, v1[ii]);
+            }
+            const uint32_t y0 = mulmod_c<SMALL>(A::reduce(a0, mp), inv[jj], mp);
+            M[r * pitch + c0] = (S)y0;
+            v0[jj] = A::from(y0, mp);
+            if (WIDE) {
+                const uint32_t y1 = mulmod_c<SMALL>(A::reduce(a1, mp), inv[jj], mp);
+                M[r * pitch + c1] = (S)y1;
+                v1[jj] = A::from(y1, mp);
+            }
+        }
+    }
+    wave_sync();
+    // every other row (rref: the panel's pivot rows too, by the later pivots
+    // only); four rows' loads are issued before their stores
+    const bool rref = op == MODP_ELIM_RREF;
+    constexpr int RB = 4;
+    for (int r0 = rref ? 0 : prow + np; r0 < m; r0 += RB) {
+        uint32_t x0[RB], x1[RB];
+#pragma unroll
+        for (int t = 0; t < RB; ++t) {
+            const int r = r0 + t;
             x0[t] = x1[t] = 0u;
-            if (l[t]) {
+            if (r < m) {
                 x0[t] = M[r * pitch + c0];
                 if (WIDE) x1[t] = M[r * pitch + c1];
             }
         }
-        wave_sync();
 #pragma unroll
-        for (int t = 0; t < RS; ++t) {
-            const int r = i + t;
-            if (l[t]) {
-                M[r * pitch + c0] = (S)row_update<SMALL>(x0[t], l[t], k0, mp);
-                if (WIDE) M[r * pitch + c1] = (S)row_update<SMALL>(x1[t], l[t], k1, mp);
+        for (int t = 0; t < RB; ++t) {
+            const int r = r0 + t;
+            if (r < m) {
+                typename A::S a0 = A::init(x0[t]), a1 = A::init(x1[t]);
+                const int own = r - prow;  // rref: the panel's pivot row own (0 <= own < np)
+#pragma unroll
+                for (int jj = 0; jj < KB; ++jj) {
+                    if (jj < np) {
+                        uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)(r < 64 ? lq0[jj] : lq1[jj]), r & 63);
+                        l = (own >= 0 && own < np && jj <= own) ? 0u : l;
+                        a0 = A::madd(a0, l, v0[jj]);
+                        if (WIDE) a1 = A::madd(a1, l, v1[jj]);
+                    }
+                }
+                M[r * pitch + c0] = (S)A::reduce(a0, mp);
+                if (WIDE) M[r * pitch + c1] = (S)A::reduce(a1, mp);
             }
         }
+    }
+    wave_sync();
+}
+
+template <typename S, int KB, int ACC>
+__device__ __forceinline__ void flush(S *M, int pitch, int m, int n, int op, int prow, int np, int pe,
+                                      const int16_t *pivcol, const uint32_t *pivinv, const ModP &mp)
+{
+    if (np <= 0) return;
+    const int lane = threadIdx.x;
+    int pc[KB];
+    uint32_t inv[KB], lq0[KB], lq1[KB];
+#pragma unroll
+    for (int jj = 0; jj < KB; ++jj) {
+        pc[jj] = 0;
+        inv[jj] = lq0[jj] = lq1[jj] = 0u;
+        if (jj < np) {
+            pc[jj] = __builtin_amdgcn_readfirstlane((int)pivcol[prow + jj]);
+            inv[jj] = (uint32_t)__builtin_amdgcn_readfirstlane((int)pivinv[prow + jj]);
+            lq0[jj] = lane < m ? (uint32_t)M[lane * pitch + pc[jj]] : 0u;
+            lq1[jj] = lane + 64 < m ? (uint32_t)M[(lane + 64) * pitch + pc[jj]] : 0u;
+        }
+    }
+    if (pe < n) {
+        if (n - pe > 64)
+            flush_cols<S, KB, ACC, true>(M, pitch, m, n, op, prow, np, pe, pc, inv, lq0, lq1, mp);
+        else
+            flush_cols<S, KB, ACC, false>(M, pitch, m, n, op, prow, np, pe, pc, inv, lq0, lq1, mp);
+    }
+    if (op == MODP_ELIM_RREF) {
+        // the multipliers have been applied: clear the pivot columns but the pivots
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj)
+            if (jj < np)
+                for (int r = lane; r < m; r += 64)
+                    if (r != prow + jj) M[r * pitch + pc[jj]] = S(0);
         wave_sync();
     }
 }
 
 // One instance on one wavefront (64 threads): argmax over rows with the lanes
-// over rows (DPP reduction of (residue, -row) keys), row swaps,
-// normalisation and the rank-1 updates with the lanes over the columns that
-// change (the multiplier of a row is one broadcast read, the pivot row sits
-// in registers).  S is the LDS element; SMALL: p < 2^16.  Every control
-// value (m, n, row, col, the pivot) is wave-uniform, so the branches are
-// scalar.
-template <typename T, typename S, bool SMALL>
+// over rows (DPP reduction of (residue, -row) keys), row swaps and pivot-row
+// normalisation with the lanes over columns, the panel's immediate updates
+// with the lanes over rows and the flushes with the lanes over columns.
+// S is the LDS element; every control value (m, n, row, col, the pivot) is
+// wave-uniform, so the branches are scalar.
+template <typename T, typename S, int KB, int ACC>
 __device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned char *esm, int bk_off)
 {
     typedef EKey<S> K;
     typedef typename K::T key_t;
+    constexpr bool SMALL = sizeof(S) < 4;
     S *M = reinterpret_cast<S *>(esm);
     uint32_t *pivval = reinterpret_cast<uint32_t *>(esm + bk_off);
-    int16_t *pivcol = reinterpret_cast<int16_t *>(pivval + EMAX);
+    uint32_t *pivinv = pivval + EMAX;
+    int16_t *pivcol = reinterpret_cast<int16_t *>(pivinv + EMAX);
     int16_t *ipiv = pivcol + EMAX;
     const int lane = threadIdx.x;
     const int m = j.m, n = j.n, pitch = j.pitch;
@@ -471,7 +585,9 @@ __device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned ch
     }
     wave_sync();
     int row = 0, col = 0, last = n - 1, nr = 0, nc = 0;
-    bool bad = false;
+    int prow = 0;               // first pivot row whose trailing update is pending
+    int pe = KB < n ? KB : n;   // end of the current panel's columns
+    bool bad = false, ended = false;
     while (row < m && col < n) {
         // largest residue of column col in rows row..m-1, lowest row on ties
         key_t mine = 0;
@@ -482,9 +598,10 @@ __device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned ch
         }
         const key_t best = wave_max(mine);
         const uint32_t bv = K::val(best);
-        if (bv == 0) {
-            if (op != MODP_ELIM_PLUQ) { ++col; continue; }
-            // pluq: is the whole block rows row.., columns col.. zero?
+        if (bv == 0 && op == MODP_ELIM_PLUQ) {
+            flush<S, KB, ACC>(M, pitch, m, n, op, prow, row - prow, pe, pivcol, pivinv, mp);
+            prow = row;
+            // is the whole block rows row.., columns col.. zero?
             int nz = 0;
             for (int c = col + lane; c < n; c += 64)
                 for (int r = row; r < m; ++r) nz |= M[r * pitch + c] != 0;
@@ -508,6 +625,7 @@ __device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned ch
                         j.col_info[2 * k + 1] = n;
                     }
                 nc += n - col;
+                ended = true;
                 break;
             }
             // swap with the last column not yet swapped; retry this column
@@ -525,48 +643,68 @@ __device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned ch
             wave_sync();
             continue;
         }
-        const uint32_t inv = modinv_fast(bv, mp);
-        if (inv == 0) { bad = true; break; }
-        const int I = K::row(best);
-        // swap rows row <-> I and normalise the pivot row from col on
-        for (int c = lane; c < n; c += 64) {
-            const uint32_t a = M[row * pitch + c], b = M[I * pitch + c];
-            M[I * pitch + c] = (S)a;
-            M[row * pitch + c] = (S)(c >= col ? mulmod_c<SMALL>(b, inv, mp) : b);
-        }
-        if (lane == 0) {
-            pivval[row] = bv;
-            pivcol[row] = (int16_t)col;
-            ipiv[row] = (int16_t)I;
-            if (op == MODP_ELIM_PLUQ && I != row) {
-                j.row_info[2 * nr] = row + 1;
-                j.row_info[2 * nr + 1] = I + 1;
+        if (bv == 0) {
+            ++col;  // lu, rref: no pivot in this column
+        } else {
+            const uint32_t inv = modinv_fast(bv, mp);
+            if (inv == 0) { bad = true; break; }
+            const int I = K::row(best);
+            // swap rows row <-> I (every column) and normalise the pivot row
+            // over the panel (the flush normalises the trailing columns)
+            for (int c = lane; c < n; c += 64) {
+                const uint32_t a = M[row * pitch + c], b = M[I * pitch + c];
+                M[I * pitch + c] = (S)a;
+                M[row * pitch + c] = (S)((c >= col && c < pe) ? mulmod_c<SMALL>(b, inv, mp) : b);
             }
+            if (lane == 0) {
+                pivval[row] = bv;
+                pivinv[row] = inv;
+                pivcol[row] = (int16_t)col;
+                ipiv[row] = (int16_t)I;
+                if (op == MODP_ELIM_PLUQ && I != row) {
+                    j.row_info[2 * nr] = row + 1;
+                    j.row_info[2 * nr + 1] = I + 1;
+                }
+            }
+            if (op == MODP_ELIM_PLUQ && I != row) ++nr;
+            wave_sync();
+            // the panel's columns col+1..pe-1, lanes over rows: rows below the
+            // pivot (pluq, lu) or every other row (rref)
+            if (col + 1 < pe) {
+                uint32_t kp[KB];
+#pragma unroll
+                for (int t = 0; t < KB; ++t) {
+                    const int c = col + 1 + t;
+                    uint32_t v = c < pe ? (uint32_t)M[row * pitch + c] : 0u;
+                    kp[t] = SMALL ? (v ? mp.p32 - v : 0u) : v;
+                }
+                const int i0 = op == MODP_ELIM_RREF ? 0 : row + 1;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int r = lane + 64 * h;
+                    if (r < m && r >= i0 && r != row) {
+                        const uint32_t l = M[r * pitch + col];
+                        if (l) {
+#pragma unroll
+                            for (int t = 0; t < KB; ++t) {
+                                const int c = col + 1 + t;
+                                if (c < pe) M[r * pitch + c] = (S)row_update<SMALL>(M[r * pitch + c], l, kp[t], mp);
+                            }
+                        }
+                    }
+                }
+                wave_sync();
+            }
+            ++row;
+            ++col;
         }
-        if (op == MODP_ELIM_PLUQ && I != row) ++nr;
-        wave_sync();
-        // the update covers columns lo..n-1: > col (pluq, lu) or >= col
-        // (rref, which also clears the rows above the pivot).  Lane q owns
-        // columns lo+q and lo+64+q (the second only while more than 64
-        // columns remain); lanes past the last column repeat it, computing
-        // and storing the same value as its owner, so no lane is masked off.
-        // The multipliers (column col) are read once, lane q holding rows q
-        // and q+64, and reach the update as scalars.
-        const int lo = op == MODP_ELIM_RREF ? col : col + 1;
-        if (lo < n) {
-            const int c0 = min(lo + lane, n - 1), c1 = min(lo + 64 + lane, n - 1);
-            const uint32_t lq0 = lane < m ? (uint32_t)M[lane * pitch + col] : 0u;
-            const uint32_t lq1 = lane + 64 < m ? (uint32_t)M[(lane + 64) * pitch + col] : 0u;
-            const int i0 = op == MODP_ELIM_RREF ? 0 : row + 1;
-            if (n - lo > 64)
-                update_rows<S, SMALL, true>(M, pitch, i0, m, row, c0, c1, lq0, lq1, mp);
-            else
-                update_rows<S, SMALL, false>(M, pitch, i0, m, row, c0, c1, lq0, lq1, mp);
+        if (col >= pe || row >= m) {
+            flush<S, KB, ACC>(M, pitch, m, n, op, prow, row - prow, pe, pivcol, pivinv, mp);
+            prow = row;
+            pe = col + KB < n ? col + KB : n;
         }
-        wave_sync();
-        ++row;
-        ++col;
     }
+    if (!bad && !ended && prow < row) flush<S, KB, ACC>(M, pitch, m, n, op, prow, row - prow, pe, pivcol, pivinv, mp);
     const int rank = row;
     // outputs, column-major: lanes over rows write consecutive elements
     T *U = (T *)j.U;
@@ -623,9 +761,9 @@ __global__ __launch_bounds__(64) void k_elim_wave(const EJob *__restrict__ jobs,
         int next = 0;
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);
         const EJob j = jobs[it];
-        if (j.cls == 0) elim_instance<T, uint8_t, true>(j, op, esm, bk_off);
-        else if (j.cls == 1) elim_instance<T, uint16_t, true>(j, op, esm, bk_off);
-        else elim_instance<T, uint32_t, false>(j, op, esm, bk_off);
+        if (j.cls == 0) elim_instance<T, uint8_t, 8, 0>(j, op, esm, bk_off);
+        else if (j.cls == 1) elim_instance<T, uint16_t, 8, 1>(j, op, esm, bk_off);
+        else elim_instance<T, uint32_t, 1, 1>(j, op, esm, bk_off);
         it = __builtin_amdgcn_readfirstlane(next);
     }
 }
