@@ -317,16 +317,30 @@ struct EJob {
     int pitch;  // LDS row pitch in elements
     int cls;    // LDS element: 0 = 8-bit (p < 2^8), 1 = 16-bit (p < 2^16), 2 = 32-bit
     ModP mp;
+    const uint16_t *itab;  // inverses mod p (prime p < 2^16), or null: extended Euclid
+};
+
+// Instances are grouped by the LDS element their modulus needs (8-bit
+// residues below 2^8, 16-bit below 2^16, else 32-bit): each class has its own
+// kernel, queue, LDS slot (sized for its largest instance) and persistent
+// grid, so the small-element classes keep their occupancy.
+constexpr int ECLASS = 3;
+struct ElimClass {
+    int64_t off = 0, count = 0;  // range of the job table
+    size_t slot = 0;             // LDS bytes per workgroup (one wavefront)
+    int bk_off = 0;              // offset of the pivot bookkeeping in the slot
+    int grid = 0;
 };
 
 struct modp_elim_plan {
     int op = 0, kind = EK_F64;
     int64_t count = 0;
     EJob *djobs = nullptr;
-    int *dcounter = nullptr;
-    size_t slot = 0;  // LDS bytes per workgroup (one wavefront)
-    int bk_off = 0;   // offset of the pivot bookkeeping in the slot
-    int grid = 0;
+    int *dcounter = nullptr;  // one queue counter per class, 64 bytes apart
+    ElimClass cls[ECLASS];
+    uint16_t *dtab = nullptr;  // inverse tables of the distinct primes below 2^16
+    hipStream_t side[ECLASS - 1] = {nullptr, nullptr};  // fork streams of the capture
+    hipEvent_t fork = nullptr, join[ECLASS - 1] = {nullptr, nullptr};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -334,7 +348,9 @@ struct modp_elim_plan {
 namespace {
 
 constexpr int EMAX = 128;
-constexpr int EBOOK = EMAX * (4 + 4 + 2 + 2);  // pivot values and inverses, pivot columns, row swaps
+// pivot values and inverses, pivot columns, row swaps, permutation, and the
+// inverse table of an 8-bit prime
+constexpr int EBOOK = EMAX * (4 + 4 + 2 + 2 + 2) + 256;
 
 // Every lane of the wavefront works on one instance, so LDS writes by one
 // lane are read by another in program order; the compiler must not move LDS
@@ -348,7 +364,7 @@ __device__ __forceinline__ void wave_sync()
 
 __global__ void k_elim_counter_reset(int *counter)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+    if (threadIdx.x < ECLASS && blockIdx.x == 0) counter[16 * threadIdx.x] = 0;
 }
 
 // argmax keys: larger key = larger residue, then lower row (rows < 256)
@@ -367,22 +383,25 @@ template <> struct EKey<uint32_t> {
 
 // Row update x - l pr (mod p) of one element.  For p < 2^16 (SMALL) the
 // caller passes c = p - pr (mod p), so t = x + l c < p + p^2 < 2^32 is
-// congruent; q = trunc(fl(t) fl(1/p)) is within one of floor(t/p) (the
-// relative error of the float quotient is below 3 2^-24 and t/p <= 2^16), so
-// r = t - q p lies in [-p, 2p) and two fix-ups finish it.  All products are
-// 24-bit integer multiplies.  Otherwise c = pr and the FP64 reduction runs
-// (|x - l pr| < 2^52 is exact).
+// congruent and reduce_small finishes it with 24-bit integer multiplies.
+// Otherwise c = pr and the FP64 reduction runs (|x - l pr| < 2^52 is exact).
+// t mod p for p < 2^16, t < 2^32 and t/p < 2^19: q = trunc(fl(fl(t) fl(1/p)
+// + 1/2)) is t/p + 1/2 truncated up to an error below 1/8 (the roundings of
+// t and 1/p contribute at most (t/p) 2^-23 < 2^-4, the fused rounding at most
+// half an ulp of a value below 2^19, 2^-5), so r = t - q p lies in
+// (-5p/8, 5p/8) and one fix-up, min(r, r + p) in unsigned arithmetic, brings
+// it into [0, p).
+__device__ __forceinline__ uint32_t reduce_small(uint32_t t, const ModP &mp)
+{
+    const uint32_t q = (uint32_t)fmaf((float)t, mp.pinvf, 0.5f);
+    const uint32_t r = t - __umul24(q, mp.p32);
+    return min(r, r + mp.p32);
+}
+
 template <bool SMALL>
 __device__ __forceinline__ uint32_t row_update(uint32_t x, uint32_t l, uint32_t c, const ModP &mp)
 {
-    if (SMALL) {
-        const uint32_t t = __umul24(l, c) + x;
-        const uint32_t q = (uint32_t)((float)t * mp.pinvf);
-        int32_t r = (int32_t)(t - __umul24(q, mp.p32));
-        r += r < 0 ? (int32_t)mp.p32 : 0;
-        r -= r >= (int32_t)mp.p32 ? (int32_t)mp.p32 : 0;
-        return (uint32_t)r;
-    }
+    if (SMALL) return reduce_small(__umul24(l, c) + x, mp);
     return (uint32_t)dmod(fma(-(double)l, (double)c, (double)x), mp);
 }
 
@@ -416,15 +435,7 @@ template <> struct Acc<0> {  // residues < 2^8, 32-bit integer sums
     typedef uint32_t S;
     static __device__ __forceinline__ S init(uint32_t x) { return x; }
     static __device__ __forceinline__ S madd(S a, uint32_t l, V v) { return a + __umul24(l, v); }
-    // a < 2^20: fl(a) is exact and the float quotient is within one
-    static __device__ __forceinline__ uint32_t reduce(S a, const ModP &mp)
-    {
-        const uint32_t q = (uint32_t)((float)a * mp.pinvf);
-        int32_t r = (int32_t)(a - __umul24(q, mp.p32));
-        r += r < 0 ? (int32_t)mp.p32 : 0;
-        r -= r >= (int32_t)mp.p32 ? (int32_t)mp.p32 : 0;
-        return (uint32_t)r;
-    }
+    static __device__ __forceinline__ uint32_t reduce(S a, const ModP &mp) { return reduce_small(a, mp); }
 };
 template <> struct Acc<1> {  // FP64 sums, exact below 2^53
     typedef double V;        // stored: v, subtracted
@@ -435,20 +446,64 @@ template <> struct Acc<1> {  // FP64 sums, exact below 2^53
     static __device__ __forceinline__ uint32_t reduce(S a, const ModP &mp) { return (uint32_t)dmod(a, mp); }
 };
 
+// Rows r0..r1-1 (all below 64, or all from 64 on: HALF picks the multiplier
+// register) of the trailing update x_r - sum_j L_{r,j} V_j.  Pivots past
+// the panel's count carry zero multipliers and values, so every row runs
+// the same KB multiply-adds; RB rows' loads are issued before their stores.
+template <typename S, int KB, int ACC, bool WIDE, int RB>
+__device__ __forceinline__ void flush_rows(S *M, int pitch, int r0, int r1, int c0, int c1, const uint32_t *lq,
+                                           const typename Acc<ACC>::V *v0, const typename Acc<ACC>::V *v1,
+                                           const ModP &mp)
+{
+    typedef Acc<ACC> A;
+    int r = r0;
+    for (; r + RB <= r1; r += RB) {
+        uint32_t x0[RB], x1[RB];
+#pragma unroll
+        for (int t = 0; t < RB; ++t) {
+            x0[t] = M[(r + t) * pitch + c0];
+            x1[t] = WIDE ? (uint32_t)M[(r + t) * pitch + c1] : 0u;
+        }
+#pragma unroll
+        for (int t = 0; t < RB; ++t) {
+            typename A::S a0 = A::init(x0[t]), a1 = A::init(x1[t]);
+#pragma unroll
+            for (int jj = 0; jj < KB; ++jj) {
+                const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)lq[jj], (r + t) & 63);
+                a0 = A::madd(a0, l, v0[jj]);
+                if (WIDE) a1 = A::madd(a1, l, v1[jj]);
+            }
+            M[(r + t) * pitch + c0] = (S)A::reduce(a0, mp);
+            if (WIDE) M[(r + t) * pitch + c1] = (S)A::reduce(a1, mp);
+        }
+    }
+    for (; r < r1; ++r) {
+        typename A::S a0 = A::init(M[r * pitch + c0]), a1 = A::init(WIDE ? (uint32_t)M[r * pitch + c1] : 0u);
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj) {
+            const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)lq[jj], r & 63);
+            a0 = A::madd(a0, l, v0[jj]);
+            if (WIDE) a1 = A::madd(a1, l, v1[jj]);
+        }
+        M[r * pitch + c0] = (S)A::reduce(a0, mp);
+        if (WIDE) M[r * pitch + c1] = (S)A::reduce(a1, mp);
+    }
+}
+
 // Trailing columns pe..n-1 of the pending pivots (rows prow..prow+np-1) on
 // lanes over columns: lane q owns columns pe+q and (WIDE) pe+64+q; lanes
 // past the last column repeat it, computing and storing the same value.
 template <typename S, int KB, int ACC, bool WIDE>
 __device__ __forceinline__ void flush_cols(S *M, int pitch, int m, int n, int op, int prow, int np, int pe,
-                                           const int *pc, const uint32_t *inv, const uint32_t *lq0,
-                                           const uint32_t *lq1, const ModP &mp)
+                                           const uint32_t *inv, const uint32_t *lq0, const uint32_t *lq1,
+                                           const ModP &mp)
 {
     typedef Acc<ACC> A;
     constexpr bool SMALL = sizeof(S) < 4;
     const int lane = threadIdx.x;
     const int c0 = min(pe + lane, n - 1), c1 = min(pe + 64 + lane, n - 1);
     typename A::V v0[KB], v1[KB];
-    // pivot rows, in order
+    // pivot rows, in order: V_j = (x_j - sum_{i<j} L_{r_j,i} V_i) inv_j
 #pragma unroll
     for (int jj = 0; jj < KB; ++jj) {
         v0[jj] = v1[jj] = typename A::V(0);
@@ -472,35 +527,26 @@ __device__ __forceinline__ void flush_cols(S *M, int pitch, int m, int n, int op
         }
     }
     wave_sync();
-    // every other row (rref: the panel's pivot rows too, by the later pivots
-    // only); four rows' loads are issued before their stores
-    const bool rref = op == MODP_ELIM_RREF;
+    // every other row
     constexpr int RB = 4;
-    for (int r0 = rref ? 0 : prow + np; r0 < m; r0 += RB) {
-        uint32_t x0[RB], x1[RB];
+    auto rows = [&](int a, int b) {
+        if (a < min(b, 64)) flush_rows<S, KB, ACC, WIDE, RB>(M, pitch, a, min(b, 64), c0, c1, lq0, v0, v1, mp);
+        if (max(a, 64) < b) flush_rows<S, KB, ACC, WIDE, RB>(M, pitch, max(a, 64), b, c0, c1, lq1, v0, v1, mp);
+    };
+    rows(prow + np, m);
+    if (op == MODP_ELIM_RREF) {
+        rows(0, prow);
+        // the panel's pivot rows take the later pivots of the panel
 #pragma unroll
-        for (int t = 0; t < RB; ++t) {
-            const int r = r0 + t;
-            x0[t] = x1[t] = 0u;
-            if (r < m) {
-                x0[t] = M[r * pitch + c0];
-                if (WIDE) x1[t] = M[r * pitch + c1];
-            }
-        }
+        for (int own = 0; own + 1 < KB; ++own) {
+            if (own + 1 < np) {
+                const int r = prow + own;
+                typename A::S a0 = A::init(M[r * pitch + c0]), a1 = A::init(WIDE ? (uint32_t)M[r * pitch + c1] : 0u);
 #pragma unroll
-        for (int t = 0; t < RB; ++t) {
-            const int r = r0 + t;
-            if (r < m) {
-                typename A::S a0 = A::init(x0[t]), a1 = A::init(x1[t]);
-                const int own = r - prow;  // rref: the panel's pivot row own (0 <= own < np)
-#pragma unroll
-                for (int jj = 0; jj < KB; ++jj) {
-                    if (jj < np) {
-                        uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)(r < 64 ? lq0[jj] : lq1[jj]), r & 63);
-                        l = (own >= 0 && own < np && jj <= own) ? 0u : l;
-                        a0 = A::madd(a0, l, v0[jj]);
-                        if (WIDE) a1 = A::madd(a1, l, v1[jj]);
-                    }
+                for (int jj = own + 1; jj < KB; ++jj) {
+                    const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)(r < 64 ? lq0[jj] : lq1[jj]), r & 63);
+                    a0 = A::madd(a0, l, v0[jj]);
+                    if (WIDE) a1 = A::madd(a1, l, v1[jj]);
                 }
                 M[r * pitch + c0] = (S)A::reduce(a0, mp);
                 if (WIDE) M[r * pitch + c1] = (S)A::reduce(a1, mp);
@@ -531,9 +577,9 @@ __device__ __forceinline__ void flush(S *M, int pitch, int m, int n, int op, int
     }
     if (pe < n) {
         if (n - pe > 64)
-            flush_cols<S, KB, ACC, true>(M, pitch, m, n, op, prow, np, pe, pc, inv, lq0, lq1, mp);
+            flush_cols<S, KB, ACC, true>(M, pitch, m, n, op, prow, np, pe, inv, lq0, lq1, mp);
         else
-            flush_cols<S, KB, ACC, false>(M, pitch, m, n, op, prow, np, pe, pc, inv, lq0, lq1, mp);
+            flush_cols<S, KB, ACC, false>(M, pitch, m, n, op, prow, np, pe, inv, lq0, lq1, mp);
     }
     if (op == MODP_ELIM_RREF) {
         // the multipliers have been applied: clear the pivot columns but the pivots
@@ -563,24 +609,44 @@ __device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned ch
     uint32_t *pivinv = pivval + EMAX;
     int16_t *pivcol = reinterpret_cast<int16_t *>(pivinv + EMAX);
     int16_t *ipiv = pivcol + EMAX;
+    int16_t *perm = ipiv + EMAX;
+    uint8_t *itab8 = reinterpret_cast<uint8_t *>(perm + EMAX);
     const int lane = threadIdx.x;
     const int m = j.m, n = j.n, pitch = j.pitch;
     const ModP &mp = j.mp;
     const T *A = (const T *)j.A;
-    // column-major input, lanes over rows (two rows per lane at most):
-    // 16 columns of loads in flight before their LDS stores
-    for (int c0 = 0; c0 < n; c0 += 16) {
-        T x0[16], x1[16];
+    // 8-bit primes: the inverse table goes to LDS (one lookup per pivot);
+    // 16-bit primes look theirs up in global memory (L2-resident: one table
+    // per prime serves the whole batch)
+    const bool tab8 = sizeof(S) == 1 && j.itab != nullptr;
+    if (tab8) {
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            const bool cv = c0 + t < n;
-            x0[t] = (cv && lane < m) ? A[lane + (int64_t)(c0 + t) * j.lda] : T(0);
-            x1[t] = (cv && lane + 64 < m) ? A[lane + 64 + (int64_t)(c0 + t) * j.lda] : T(0);
+        for (int t = 0; t < 4; ++t) {
+            const int e = 4 * lane + t;
+            itab8[e] = (uint8_t)(e < (int)mp.p32 ? j.itab[e] : 0u);
         }
+    }
+    // column-major input, lanes over rows (two rows per lane at most):
+    // 16 columns of loads in flight before their LDS stores.  Lanes past the
+    // last row or column repeat it, loading and storing the same value as
+    // its owner, so no lane is masked off.
+    {
+        const int r0 = min(lane, m - 1), r1 = min(lane + 64, m - 1);
+        const bool two = m > 64;
+        for (int c0 = 0; c0 < n; c0 += 16) {
+            T x0[16], x1[16];
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            if (c0 + t < n && lane < m) M[lane * pitch + c0 + t] = (S)to_res<T>(x0[t], mp);
-            if (c0 + t < n && lane + 64 < m) M[(lane + 64) * pitch + c0 + t] = (S)to_res<T>(x1[t], mp);
+            for (int t = 0; t < 16; ++t) {
+                const int64_t c = min(c0 + t, n - 1);
+                x0[t] = A[r0 + c * j.lda];
+                x1[t] = two ? A[r1 + c * j.lda] : T(0);
+            }
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int c = min(c0 + t, n - 1);
+                M[r0 * pitch + c] = (S)to_res<T>(x0[t], mp);
+                if (two) M[r1 * pitch + c] = (S)to_res<T>(x1[t], mp);
+            }
         }
     }
     wave_sync();
@@ -588,13 +654,21 @@ __device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned ch
     int prow = 0;               // first pivot row whose trailing update is pending
     int pe = KB < n ? KB : n;   // end of the current panel's columns
     bool bad = false, ended = false;
+    // the panel update computes the next pivot column's keys on the way
+    key_t pre = 0;
+    bool have_pre = false;
     while (row < m && col < n) {
         // largest residue of column col in rows row..m-1, lowest row on ties
         key_t mine = 0;
-        if (lane >= row && lane < m) mine = K::make(M[lane * pitch + col], lane);
-        if (lane + 64 >= row && lane + 64 < m) {
-            const key_t k = K::make(M[(lane + 64) * pitch + col], lane + 64);
-            mine = k > mine ? k : mine;
+        if (have_pre) {
+            mine = pre;
+            have_pre = false;
+        } else {
+            if (lane >= row && lane < m) mine = K::make(M[lane * pitch + col], lane);
+            if (lane + 64 >= row && lane + 64 < m) {
+                const key_t k = K::make(M[(lane + 64) * pitch + col], lane + 64);
+                mine = k > mine ? k : mine;
+            }
         }
         const key_t best = wave_max(mine);
         const uint32_t bv = K::val(best);
@@ -646,15 +720,23 @@ __device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned ch
         if (bv == 0) {
             ++col;  // lu, rref: no pivot in this column
         } else {
-            const uint32_t inv = modinv_fast(bv, mp);
+            const uint32_t inv = tab8 ? (uint32_t)itab8[bv]
+                                      : (sizeof(S) == 2 && j.itab ? (uint32_t)j.itab[bv] : modinv_fast(bv, mp));
             if (inv == 0) { bad = true; break; }
             const int I = K::row(best);
             // swap rows row <-> I (every column) and normalise the pivot row
             // over the panel (the flush normalises the trailing columns)
-            for (int c = lane; c < n; c += 64) {
-                const uint32_t a = M[row * pitch + c], b = M[I * pitch + c];
-                M[I * pitch + c] = (S)a;
-                M[row * pitch + c] = (S)((c >= col && c < pe) ? mulmod_c<SMALL>(b, inv, mp) : b);
+            {
+                const int ca = min(lane, n - 1), cb = min(lane + 64, n - 1);
+                const uint32_t a0 = M[row * pitch + ca], b0 = M[I * pitch + ca];
+                const uint32_t a1 = M[row * pitch + cb], b1 = M[I * pitch + cb];
+                wave_sync();
+                M[I * pitch + ca] = (S)a0;
+                M[row * pitch + ca] = (S)((ca >= col && ca < pe) ? mulmod_c<SMALL>(b0, inv, mp) : b0);
+                if (n > 64) {
+                    M[I * pitch + cb] = (S)a1;
+                    M[row * pitch + cb] = (S)((cb >= col && cb < pe) ? mulmod_c<SMALL>(b1, inv, mp) : b1);
+                }
             }
             if (lane == 0) {
                 pivval[row] = bv;
@@ -671,25 +753,40 @@ __device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned ch
             // the panel's columns col+1..pe-1, lanes over rows: rows below the
             // pivot (pluq, lu) or every other row (rref)
             if (col + 1 < pe) {
-                uint32_t kp[KB];
-#pragma unroll
-                for (int t = 0; t < KB; ++t) {
-                    const int c = col + 1 + t;
-                    uint32_t v = c < pe ? (uint32_t)M[row * pitch + c] : 0u;
-                    kp[t] = SMALL ? (v ? mp.p32 - v : 0u) : v;
-                }
+                // lane q owns rows q and q+64; lanes past the last row repeat
+                // it (same multiplier, same values, same stores).  All loads
+                // are issued before the updates; reads past column pe-1 stay
+                // inside the slot (the bookkeeping follows the matrix), only
+                // the panel is written.
                 const int i0 = op == MODP_ELIM_RREF ? 0 : row + 1;
+                const int ra = min(lane, m - 1), rb = min(lane + 64, m - 1);
+                const bool two = m > 64;
+                uint32_t kp[KB], xa[KB], xb[KB];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int r = lane + 64 * h;
-                    if (r < m && r >= i0 && r != row) {
-                        const uint32_t l = M[r * pitch + col];
-                        if (l) {
+                for (int t = 0; t + 1 < KB; ++t) {
+                    const uint32_t v = M[row * pitch + col + 1 + t];
+                    kp[t] = SMALL ? (v ? mp.p32 - v : 0u) : v;
+                    xa[t] = M[ra * pitch + col + 1 + t];
+                    xb[t] = two ? (uint32_t)M[rb * pitch + col + 1 + t] : 0u;
+                }
+                const uint32_t la = (ra >= i0 && ra != row) ? (uint32_t)M[ra * pitch + col] : 0u;
+                const uint32_t lb = (two && rb >= i0 && rb != row) ? (uint32_t)M[rb * pitch + col] : 0u;
+                wave_sync();
 #pragma unroll
-                            for (int t = 0; t < KB; ++t) {
-                                const int c = col + 1 + t;
-                                if (c < pe) M[r * pitch + c] = (S)row_update<SMALL>(M[r * pitch + c], l, kp[t], mp);
+                for (int t = 0; t + 1 < KB; ++t) {
+                    if (col + 1 + t < pe) {
+                        const uint32_t ya = row_update<SMALL>(xa[t], la, kp[t], mp);
+                        const uint32_t yb = two ? row_update<SMALL>(xb[t], lb, kp[t], mp) : 0u;
+                        M[ra * pitch + col + 1 + t] = (S)ya;
+                        if (two) M[rb * pitch + col + 1 + t] = (S)yb;
+                        if (t == 0) {
+                            // keys of column col+1 over rows row+1.. for the next argmax
+                            pre = ra > row ? K::make(ya, ra) : key_t(0);
+                            if (two && rb > row) {
+                                const key_t kb = K::make(yb, rb);
+                                pre = kb > pre ? kb : pre;
                             }
+                            have_pre = true;
                         }
                     }
                 }
@@ -706,39 +803,80 @@ __device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned ch
     }
     if (!bad && !ended && prow < row) flush<S, KB, ACC>(M, pitch, m, n, op, prow, row - prow, pe, pivcol, pivinv, mp);
     const int rank = row;
-    // outputs, column-major: lanes over rows write consecutive elements
-    T *U = (T *)j.U;
-    for (int c = 0; c < n; ++c)
-        for (int r = lane; r < m; r += 64) {
-            uint32_t v = 0;
-            if (op == MODP_ELIM_RREF) v = r < rank ? (uint32_t)M[r * pitch + c] : 0u;
-            else if (r < rank && c >= pivcol[r]) v = M[r * pitch + c];
-            U[r + (int64_t)c * j.ldu] = from_res<T>(v);
-        }
-    if (op != MODP_ELIM_RREF) {
-        T *L = (T *)j.L;
-        for (int k = 0; k < m; ++k) {
-            const int pc = k < rank ? pivcol[k] : 0;
-            const uint32_t pv = k < rank ? pivval[k] : 0u;
-            for (int r = lane; r < m; r += 64) {
-                uint32_t v = 0;
-                if (k < rank) v = r == k ? pv : (r > k ? (uint32_t)M[r * pitch + pc] : 0u);
-                L[r + (int64_t)k * j.ldl] = from_res<T>(v);
+    // outputs, column-major: lanes over rows write consecutive elements,
+    // 16 columns of LDS reads before their stores; lanes past the last row
+    // or column repeat it (same address, same value)
+    const int r0 = min(lane, m - 1), r1 = min(lane + 64, m - 1);
+    const bool two = m > 64;
+    {
+        T *U = (T *)j.U;
+        // first column of row r's U part (n: none)
+        const int f0 = r0 < rank ? (op == MODP_ELIM_RREF ? 0 : (int)pivcol[r0]) : n;
+        const int f1 = r1 < rank ? (op == MODP_ELIM_RREF ? 0 : (int)pivcol[r1]) : n;
+        for (int c0 = 0; c0 < n; c0 += 16) {
+            uint32_t x0[16], x1[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int c = min(c0 + t, n - 1);
+                x0[t] = M[r0 * pitch + c];
+                x1[t] = two ? (uint32_t)M[r1 * pitch + c] : 0u;
+            }
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int c = min(c0 + t, n - 1);
+                U[r0 + (int64_t)c * j.ldu] = from_res<T>(c >= f0 ? x0[t] : 0u);
+                if (two) U[r1 + (int64_t)c * j.ldu] = from_res<T>(c >= f1 ? x1[t] : 0u);
             }
         }
     }
-    if (lane == 0) {
-        if (op == MODP_ELIM_LU) {
-            for (int q = 0; q < m; ++q) j.row_info[q] = q + 1;
-            for (int q = 0; q < rank; ++q) {
-                const int64_t t = j.row_info[q];
-                j.row_info[q] = j.row_info[ipiv[q]];
-                j.row_info[ipiv[q]] = t;
+    if (op != MODP_ELIM_RREF) {
+        // L: column k < rank holds the pivot value on the diagonal and the
+        // multipliers (pivot column k) below it
+        T *L = (T *)j.L;
+        for (int k0 = 0; k0 < m; k0 += 16) {
+            int pc[16];
+            uint32_t pv[16], x0[16], x1[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int kk = min(min(k0 + t, m - 1), rank > 0 ? rank - 1 : 0);
+                pc[t] = rank > 0 ? (int)pivcol[kk] : 0;
+                pv[t] = rank > 0 ? pivval[kk] : 0u;
             }
-            for (int q = 0; q < rank; ++q) j.col_info[q] = pivcol[q] + 1;
-        } else if (op == MODP_ELIM_RREF) {
-            for (int q = 0; q < rank; ++q) j.row_info[q] = pivcol[q] + 1;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                x0[t] = M[r0 * pitch + pc[t]];
+                x1[t] = two ? (uint32_t)M[r1 * pitch + pc[t]] : 0u;
+            }
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int k = min(k0 + t, m - 1);
+                const uint32_t v0 = k < rank ? (r0 == k ? pv[t] : (r0 > k ? x0[t] : 0u)) : 0u;
+                L[r0 + (int64_t)k * j.ldl] = from_res<T>(v0);
+                if (two) {
+                    const uint32_t v1 = k < rank ? (r1 == k ? pv[t] : (r1 > k ? x1[t] : 0u)) : 0u;
+                    L[r1 + (int64_t)k * j.ldl] = from_res<T>(v1);
+                }
+            }
         }
+    }
+    if (op == MODP_ELIM_LU) {
+        // row permutation: the identity with the pivot swaps applied in order
+        // (built in LDS, written by all lanes)
+        if (lane == 0) {
+            for (int q = 0; q < m; ++q) perm[q] = (int16_t)q;
+            for (int q = 0; q < rank; ++q) {
+                const int16_t t = perm[q];
+                perm[q] = perm[ipiv[q]];
+                perm[ipiv[q]] = t;
+            }
+        }
+        wave_sync();
+        for (int q = lane; q < m; q += 64) j.row_info[q] = perm[q] + 1;
+        for (int q = lane; q < rank; q += 64) j.col_info[q] = pivcol[q] + 1;
+    } else if (op == MODP_ELIM_RREF) {
+        for (int q = lane; q < rank; q += 64) j.row_info[q] = pivcol[q] + 1;
+    }
+    if (lane == 0) {
         j.counts[0] = op == MODP_ELIM_PLUQ ? nr : 0;
         j.counts[1] = op == MODP_ELIM_PLUQ ? nc : 0;
         j.counts[2] = bad ? -1 : rank;
@@ -746,39 +884,47 @@ __device__ __forceinline__ void elim_instance(const EJob &j, int op, unsigned ch
     wave_sync();
 }
 
-// Persistent kernel: each single-wavefront workgroup takes instances from
-// one device queue (the job table is sorted by cost, largest first), the
-// next index fetched while the current instance runs.
-template <typename T>
+// Persistent kernel of one class: workgroup w (one wavefront) starts with
+// instance w of the class (sorted by cost, largest first; the grid never
+// exceeds the count), then takes instances grid, grid+1, ... from the
+// class's queue as it finishes.  Taking the next index only after an
+// instance is done keeps a workgroup that starts early from claiming two
+// large instances while later workgroups are still being dispatched.
+template <typename T, int CLS>
 __global__ __launch_bounds__(64) void k_elim_wave(const EJob *__restrict__ jobs, int64_t njobs, int op,
                                                   int *__restrict__ counter, int bk_off)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char esm[];
-    int it = 0;
-    if (threadIdx.x == 0) it = atomicAdd(counter, 1);
-    it = __builtin_amdgcn_readfirstlane(it);
+    int64_t it = blockIdx.x;
     while (it < njobs) {
+        const EJob j = jobs[it];
+        if (CLS == 0) elim_instance<T, uint8_t, 8, 0>(j, op, esm, bk_off);
+        else if (CLS == 1) elim_instance<T, uint16_t, 8, 1>(j, op, esm, bk_off);
+        else elim_instance<T, uint32_t, 1, 1>(j, op, esm, bk_off);
         int next = 0;
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);
-        const EJob j = jobs[it];
-        if (j.cls == 0) elim_instance<T, uint8_t, 8, 0>(j, op, esm, bk_off);
-        else if (j.cls == 1) elim_instance<T, uint16_t, 8, 1>(j, op, esm, bk_off);
-        else elim_instance<T, uint32_t, 1, 1>(j, op, esm, bk_off);
-        it = __builtin_amdgcn_readfirstlane(next);
+        it = (int64_t)gridDim.x + __builtin_amdgcn_readfirstlane(next);
     }
 }
 
-template <typename T> hipError_t allow_elim_slot(size_t slot)
+template <typename T, int CLS> hipError_t class_kernel(const modp_elim_plan *pl, hipStream_t st, bool allow)
 {
+    const ElimClass &ec = pl->cls[CLS];
     // the dynamic LDS can exceed the default 64 KiB (32-bit residues of
     // 128 x 128 instances)
-    return hipFuncSetAttribute((const void *)k_elim_wave<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slot);
+    if (allow)
+        return hipFuncSetAttribute((const void *)k_elim_wave<T, CLS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)ec.slot);
+    k_elim_wave<T, CLS><<<(unsigned)ec.grid, 64, ec.slot, st>>>(pl->djobs + ec.off, ec.count, pl->op,
+                                                                 pl->dcounter + 16 * CLS, ec.bk_off);
+    return hipGetLastError();
 }
 
-template <typename T> hipError_t launch_elim(const modp_elim_plan *pl, hipStream_t st)
+template <typename T> hipError_t class_launch(const modp_elim_plan *pl, int c, hipStream_t st, bool allow)
 {
-    k_elim_wave<T><<<(unsigned)pl->grid, 64, pl->slot, st>>>(pl->djobs, pl->count, pl->op, pl->dcounter, pl->bk_off);
-    return hipGetLastError();
+    if (c == 0) return class_kernel<T, 0>(pl, st, allow);
+    if (c == 1) return class_kernel<T, 1>(pl, st, allow);
+    return class_kernel<T, 2>(pl, st, allow);
 }
 
 int elim_class(int64_t p) { return p < 256 ? 0 : (p < 65536 ? 1 : 2); }
@@ -800,16 +946,24 @@ int batched_elim_destroy(modp_ctx *ctx, modp_elim_plan *pl)
     (void)hipStreamSynchronize(ctx->stream);
     if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    for (int i = 0; i < ECLASS - 1; ++i) {
+        if (pl->join[i]) (void)hipEventDestroy(pl->join[i]);
+        if (pl->side[i]) (void)hipStreamDestroy(pl->side[i]);
+    }
+    if (pl->fork) (void)hipEventDestroy(pl->fork);
+    if (pl->dtab) (void)hipFree(pl->dtab);
     if (pl->djobs) (void)hipFree(pl->djobs);
     if (pl->dcounter) (void)hipFree(pl->dcounter);
     delete pl;
     return MODP_OK;
 }
 
-// The plan sorts the instances by cost (largest first, so the queue ends
-// with short ones), gives each its LDS element and row pitch, sizes the LDS
-// slot for the largest, uploads the job table once and captures
-// [queue reset, one persistent kernel] in a hipGraph.
+// The plan groups the instances by class, sorts each class by cost (largest
+// first, so its queue ends with short ones), gives each instance its row
+// pitch, sizes each class's LDS slot for its largest instance, uploads the
+// job table once and captures [queue reset, one persistent kernel per class]
+// in a hipGraph, the classes on parallel branches (forked streams) so they
+// run side by side.
 int batched_elim_plan(modp_ctx *ctx, int op, int kind, const modp_elim_job *jobs, int64_t count,
                       modp_elim_plan **out)
 {
@@ -820,48 +974,103 @@ int batched_elim_plan(modp_ctx *ctx, int op, int kind, const modp_elim_job *jobs
     pl->count = count;
     std::vector<EJob> hj;
     hj.reserve((size_t)(count > 0 ? count : 0));
-    // cost: multiply-adds of the updates, the 32-bit class at twice the rate
     auto cost = [&](int64_t i) {
         const int64_t a = jobs[i].m, b = jobs[i].n;
-        return a * b * (a < b ? a : b) * (elim_class(jobs[i].p) == 2 ? 2 : 1);
+        return a * b * (a < b ? a : b);
     };
-    std::vector<int64_t> order((size_t)(count > 0 ? count : 0));
-    std::iota(order.begin(), order.end(), (int64_t)0);
-    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
-    size_t mat_bytes = 16;
-    for (int64_t oi : order) {
-        const modp_elim_job &s = jobs[oi];
-        EJob d;
-        d.A = s.A; d.lda = s.lda; d.U = s.U; d.ldu = s.ldu; d.L = s.L; d.ldl = s.ldl;
-        d.row_info = s.row_info; d.col_info = s.col_info; d.counts = s.counts;
-        d.m = (int)s.m; d.n = (int)s.n;
-        d.cls = elim_class(s.p);
-        const int eb = d.cls == 0 ? 1 : (d.cls == 1 ? 2 : 4);
-        d.pitch = elim_pitch(d.n, eb);
-        d.mp = make_modp(s.p);
-        mat_bytes = std::max(mat_bytes, (size_t)d.m * (size_t)d.pitch * (size_t)eb);
-        hj.push_back(d);
+    // inverse tables for the distinct primes below 2^16 (up to 16 MiB of
+    // tables; other moduli keep the extended Euclidean algorithm), by
+    // inv(i) = -(p div i) inv(p mod i) mod p
+    std::map<int64_t, size_t> tab_off;
+    std::vector<uint16_t> htab;
+    for (int64_t i = 0; i < count; ++i) {
+        const int64_t q = jobs[i].p;
+        if (q >= 65536 || q < 2 || tab_off.count(q)) continue;
+        bool prime = true;
+        for (int64_t d = 2; d * d <= q && prime; ++d) prime = q % d != 0;
+        if (!prime || htab.size() + (size_t)q > ((size_t)8 << 20)) continue;
+        tab_off[q] = htab.size();
+        htab.resize(htab.size() + (size_t)q + (q & 1));  // keep 4-byte alignment
+        uint16_t *t = htab.data() + tab_off[q];
+        t[0] = 0;
+        if (q > 1) t[1] = 1;
+        for (int64_t a = 2; a < q; ++a) t[a] = (uint16_t)((q - (q / a) * (int64_t)t[q % a] % q) % q);
     }
-    pl->bk_off = (int)((mat_bytes + 15) & ~(size_t)15);
-    pl->slot = (size_t)pl->bk_off + EBOOK;
-    // persistent grid: as many single-wavefront workgroups as the LDS holds
-    const size_t lds_cu = 160 * 1024;
-    int64_t per_cu = (int64_t)(lds_cu / pl->slot);
-    if (per_cu > 16) per_cu = 16;
-    if (per_cu < 1) per_cu = 1;
-    int64_t g = per_cu * ctx->num_cus;
-    if (g > count) g = count;
-    pl->grid = (int)(g > 0 ? g : 1);
-    hipError_t e = hipMalloc(&pl->djobs, sizeof(EJob) * (count > 0 ? count : 1));
+    hipError_t e = hipSuccess;
+    if (!htab.empty()) {
+        e = hipMalloc(&pl->dtab, htab.size() * sizeof(uint16_t));
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(pl->dtab, htab.data(), htab.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
+                               ctx->stream);
+    }
+    for (int c = 0; c < ECLASS; ++c) {
+        std::vector<int64_t> order;
+        for (int64_t i = 0; i < count; ++i)
+            if (elim_class(jobs[i].p) == c) order.push_back(i);
+        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
+        ElimClass &ec = pl->cls[c];
+        ec.off = (int64_t)hj.size();
+        ec.count = (int64_t)order.size();
+        const int eb = c == 0 ? 1 : (c == 1 ? 2 : 4);
+        size_t mat_bytes = 16;
+        for (int64_t oi : order) {
+            const modp_elim_job &s = jobs[oi];
+            EJob d;
+            d.A = s.A; d.lda = s.lda; d.U = s.U; d.ldu = s.ldu; d.L = s.L; d.ldl = s.ldl;
+            d.row_info = s.row_info; d.col_info = s.col_info; d.counts = s.counts;
+            d.m = (int)s.m; d.n = (int)s.n;
+            d.cls = c;
+            d.pitch = elim_pitch(d.n, eb);
+            d.mp = make_modp(s.p);
+            const auto ti = tab_off.find(s.p);
+            d.itab = ti == tab_off.end() ? nullptr : pl->dtab + ti->second;
+            mat_bytes = std::max(mat_bytes, (size_t)d.m * (size_t)d.pitch * (size_t)eb);
+            hj.push_back(d);
+        }
+        ec.bk_off = (int)((mat_bytes + 15) & ~(size_t)15);
+        ec.slot = (size_t)ec.bk_off + EBOOK;
+        // persistent grid: as many single-wavefront workgroups as the LDS holds
+        const size_t lds_cu = 160 * 1024;
+        int64_t per_cu = (int64_t)(lds_cu / ec.slot);
+        if (per_cu > 16) per_cu = 16;
+        if (per_cu < 1) per_cu = 1;
+        const int64_t g = per_cu * ctx->num_cus;
+        ec.grid = (int)(g < ec.count ? g : ec.count);
+    }
+    if (e == hipSuccess) e = hipMalloc(&pl->djobs, sizeof(EJob) * (count > 0 ? count : 1));
     if (e == hipSuccess) e = hipMalloc(&pl->dcounter, 256);
     if (e == hipSuccess && count > 0)
         e = hipMemcpyAsync(pl->djobs, hj.data(), sizeof(EJob) * count, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // hj is released on return
-    if (e == hipSuccess) e = kind == EK_F64 ? allow_elim_slot<double>(pl->slot) : allow_elim_slot<float>(pl->slot);
+    int act[ECLASS], na = 0;  // classes with work, the largest total cost first
+    for (int c = ECLASS - 1; c >= 0; --c)
+        if (pl->cls[c].count > 0) act[na++] = c;
+    for (int i = 0; i < na && e == hipSuccess; ++i)
+        e = kind == EK_F64 ? class_launch<double>(pl, act[i], ctx->stream, true)
+                           : class_launch<float>(pl, act[i], ctx->stream, true);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->fork, hipEventDisableTiming);
+    for (int i = 0; i + 1 < na && e == hipSuccess; ++i) {
+        e = hipStreamCreateWithFlags(&pl->side[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->join[i], hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
+        hipError_t le = hipSuccess;
         k_elim_counter_reset<<<1, 64, 0, ctx->stream>>>(pl->dcounter);
-        const hipError_t le = kind == EK_F64 ? launch_elim<double>(pl, ctx->stream) : launch_elim<float>(pl, ctx->stream);
+        le = hipGetLastError();
+        if (le == hipSuccess && na > 1) le = hipEventRecord(pl->fork, ctx->stream);
+        for (int i = 1; i < na && le == hipSuccess; ++i) {
+            hipStream_t st = pl->side[i - 1];
+            le = hipStreamWaitEvent(st, pl->fork, 0);
+            if (le == hipSuccess)
+                le = kind == EK_F64 ? class_launch<double>(pl, act[i], st, false)
+                                    : class_launch<float>(pl, act[i], st, false);
+            if (le == hipSuccess) le = hipEventRecord(pl->join[i - 1], st);
+        }
+        if (le == hipSuccess && na > 0)
+            le = kind == EK_F64 ? class_launch<double>(pl, act[0], ctx->stream, false)
+                                : class_launch<float>(pl, act[0], ctx->stream, false);
+        for (int i = 1; i < na && le == hipSuccess; ++i) le = hipStreamWaitEvent(ctx->stream, pl->join[i - 1], 0);
         e = hipStreamEndCapture(ctx->stream, &pl->graph);
         if (e == hipSuccess) e = le;
     }
