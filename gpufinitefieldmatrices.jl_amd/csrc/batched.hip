@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <map>
 #include <numeric>
+#include <type_traits>
 
 #include "batched.h"
 #include "gemm_tile.h"
@@ -490,13 +491,148 @@ __device__ __forceinline__ void flush_rows(S *M, int pitch, int r0, int r1, int 
     }
 }
 
+typedef int ev4i __attribute__((ext_vector_type(4)));
+typedef int ev16i __attribute__((ext_vector_type(16)));
+
+// Rows a..b-1 of the trailing update for 8-bit residues on int8 MFMA
+// (v_mfma_i32_32x32x32_i8), one 32 x 32 tile of rows x trailing columns per
+// instruction: D = x + sum_j L[r][j] W[j][c] with L the multipliers (pivot
+// columns pc[j]) and W = -V (the pivot rows' trailing values, written by the
+// flush's first step), both as balanced residues in (-p/2, p/2] (|.| <= 125),
+// so |sum| <= 8 * 125^2 and D fits int32.  Pivots past np contribute zeros;
+// K bytes 8..31 of the instruction are zero.  Operand layouts (lane l): A row
+// l % 32 and B column l % 32 with K bytes 16 (l / 32) .. +15; accumulator
+// column l % 32, rows 8 (e / 4) + 4 (l / 32) + e % 4.  Rows and columns past
+// the range repeat the last one (same inputs, same value stored).
+// D mod p: q = rint(fl(D) fl(1/p)) is within 0.01 of D/p (|D|/p < 2^16), so
+// r = D - q p lies in (-0.51 p, 0.51 p) and one fix-up, min(r, r + p) in
+// unsigned arithmetic, finishes it.
+template <int KB>
+__device__ __forceinline__ void flush_rows_mfma8(uint8_t *M, int pitch, int a, int b, int pe, int n, int prow,
+                                                 int np, const int *pc, const ModP &mp)
+{
+    if (a >= b) return;
+    const int lane = threadIdx.x, l32 = lane & 31, hi = lane >> 5;
+    const int p = (int)mp.p32, half = p >> 1;
+    const int ntc = (n - pe + 31) >> 5;  // column tiles, at most 4
+    ev4i bop[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        uint32_t w[2] = {0u, 0u};
+        if (ct < ntc) {
+            const int c = min(pe + 32 * ct + l32, n - 1);
+#pragma unroll
+            for (int jj = 0; jj < KB; ++jj) {
+                if (jj < np) {
+                    const int v = M[(prow + jj) * pitch + c];
+                    const int nv = v > half ? p - v : -v;  // -v, balanced
+                    w[jj >> 2] |= ((uint32_t)nv & 255u) << (8 * (jj & 3));
+                }
+            }
+        }
+        bop[ct] = hi ? ev4i{0, 0, 0, 0} : ev4i{(int)w[0], (int)w[1], 0, 0};
+    }
+    for (int r0 = a; r0 < b; r0 += 32) {
+        uint32_t w[2] = {0u, 0u};
+        const int rr = min(r0 + l32, b - 1);
+#pragma unroll
+        for (int jj = 0; jj < KB; ++jj) {
+            if (jj < np) {
+                const int v = M[rr * pitch + pc[jj]];
+                const int bv = v > half ? v - p : v;  // balanced
+                w[jj >> 2] |= ((uint32_t)bv & 255u) << (8 * (jj & 3));
+            }
+        }
+        const ev4i aop = hi ? ev4i{0, 0, 0, 0} : ev4i{(int)w[0], (int)w[1], 0, 0};
+        int ro[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) ro[e] = min(r0 + 8 * (e >> 2) + 4 * hi + (e & 3), b - 1) * pitch;
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            if (ct < ntc) {
+                const int c = min(pe + 32 * ct + l32, n - 1);
+                ev16i acc;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[e] = M[ro[e] + c];
+                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop, bop[ct], acc, 0, 0, 0);
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    // q = round(d / p): adding 1.5 2^23 leaves the rounded
+                    // quotient in the low mantissa bits (|q| < 2^17)
+                    const int d = acc[e];
+                    const int q = __float_as_int(fmaf((float)d, mp.pinvf, 12582912.0f)) - 0x4B400000;
+                    const uint32_t r = (uint32_t)(d - __mul24(q, p));
+                    M[ro[e] + c] = (uint8_t)min(r, r + (uint32_t)p);
+                }
+            }
+        }
+    }
+}
+
+// Rows a..b-1 of the trailing update for 16-bit residues on FP64 MFMA
+// (v_mfma_f64_16x16x4f64, two per 16 x 16 tile for the eight pivots):
+// D = x - sum_j L[r][j] V[j][c] with |sum| < 8 2^32, exact in FP64, then
+// reduced mod p.  Operand layouts (lane l): A row l & 15 and B column l & 15
+// with K index l >> 4; accumulator column l & 15, rows (l >> 4) + 4 e.  Rows
+// and columns past the range repeat the last one.  V comes from the pivot
+// rows' trailing columns, written by the flush's first step.
+template <int KB>
+__device__ __forceinline__ void flush_rows_mfma16(uint16_t *M, int pitch, int a, int b, int pe, int n, int prow,
+                                                  int np, const int16_t *pivcol, const ModP &mp)
+{
+    static_assert(KB == 8, "two K=4 steps per tile");
+    if (a >= b) return;
+    const int lane = threadIdx.x, l16 = lane & 15, kq = lane >> 4;
+    const int ntc = (n - pe + 15) >> 4;  // column tiles, at most 8
+    // this lane's pivots kq and kq + 4 (their columns from the bookkeeping)
+    const bool ka = kq < np, kb = kq + 4 < np;
+    const int pca = ka ? (int)pivcol[prow + kq] : 0;
+    const int pcb = kb ? (int)pivcol[prow + kq + 4] : 0;
+    const int va = (prow + (ka ? kq : 0)) * pitch, vb = (prow + (kb ? kq + 4 : 0)) * pitch;
+    for (int r0 = a; r0 < b; r0 += 16) {
+        const int rr = min(r0 + l16, b - 1);
+        const double a0 = ka ? (double)M[rr * pitch + pca] : 0.0;
+        const double a1 = kb ? (double)M[rr * pitch + pcb] : 0.0;
+        int ro[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ro[e] = min(r0 + kq + 4 * e, b - 1) * pitch;
+        // G column tiles at a time: all their loads, then the products, then
+        // the reductions and stores (tiles past the last repeat it)
+        auto tiles = [&](int ct0, auto G) {
+            constexpr int NG = decltype(G)::value;
+            v4d acc[NG];
+            double b0[NG], b1[NG];
+            int cc[NG];
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                cc[g] = min(pe + 16 * (ct0 + g) + l16, n - 1);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[g][e] = (double)M[ro[e] + cc[g]];
+                b0[g] = ka ? -(double)M[va + cc[g]] : 0.0;
+                b1[g] = kb ? -(double)M[vb + cc[g]] : 0.0;
+            }
+#pragma unroll
+            for (int g = 0; g < NG; ++g) acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0[g], acc[g], 0, 0, 0);
+#pragma unroll
+            for (int g = 0; g < NG; ++g) acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1[g], acc[g], 0, 0, 0);
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) M[ro[e] + cc[g]] = (uint16_t)dmod(acc[g][e], mp);
+        };
+        int ct = 0;
+        for (; ct + 4 <= ntc; ct += 4) tiles(ct, std::integral_constant<int, 4>());
+        for (; ct < ntc; ++ct) tiles(ct, std::integral_constant<int, 1>());
+    }
+}
+
 // Trailing columns pe..n-1 of the pending pivots (rows prow..prow+np-1) on
 // lanes over columns: lane q owns columns pe+q and (WIDE) pe+64+q; lanes
 // past the last column repeat it, computing and storing the same value.
 template <typename S, int KB, int ACC, bool WIDE>
 __device__ __forceinline__ void flush_cols(S *M, int pitch, int m, int n, int op, int prow, int np, int pe,
-                                           const uint32_t *inv, const uint32_t *lq0, const uint32_t *lq1,
-                                           const ModP &mp)
+                                           const int *pc, const int16_t *pivcol, const uint32_t *inv,
+                                           const uint32_t *lq0, const uint32_t *lq1, const ModP &mp)
 {
     typedef Acc<ACC> A;
     constexpr bool SMALL = sizeof(S) < 4;
@@ -533,9 +669,19 @@ __device__ __forceinline__ void flush_cols(S *M, int pitch, int m, int n, int op
         if (a < min(b, 64)) flush_rows<S, KB, ACC, WIDE, RB>(M, pitch, a, min(b, 64), c0, c1, lq0, v0, v1, mp);
         if (max(a, 64) < b) flush_rows<S, KB, ACC, WIDE, RB>(M, pitch, max(a, 64), b, c0, c1, lq1, v0, v1, mp);
     };
-    rows(prow + np, m);
+    if (sizeof(S) == 1) {
+        uint8_t *M8 = reinterpret_cast<uint8_t *>(M);
+        flush_rows_mfma8<KB>(M8, pitch, prow + np, m, pe, n, prow, np, pc, mp);
+        if (op == MODP_ELIM_RREF) flush_rows_mfma8<KB>(M8, pitch, 0, prow, pe, n, prow, np, pc, mp);
+    } else if (sizeof(S) == 2 && KB == 8) {
+        uint16_t *M16 = reinterpret_cast<uint16_t *>(M);
+        flush_rows_mfma16<8>(M16, pitch, prow + np, m, pe, n, prow, np, pivcol, mp);
+        if (op == MODP_ELIM_RREF) flush_rows_mfma16<8>(M16, pitch, 0, prow, pe, n, prow, np, pivcol, mp);
+    } else {
+        rows(prow + np, m);
+        if (op == MODP_ELIM_RREF) rows(0, prow);
+    }
     if (op == MODP_ELIM_RREF) {
-        rows(0, prow);
         // the panel's pivot rows take the later pivots of the panel
 #pragma unroll
         for (int own = 0; own + 1 < KB; ++own) {
@@ -577,9 +723,9 @@ __device__ __forceinline__ void flush(S *M, int pitch, int m, int n, int op, int
     }
     if (pe < n) {
         if (n - pe > 64)
-            flush_cols<S, KB, ACC, true>(M, pitch, m, n, op, prow, np, pe, inv, lq0, lq1, mp);
+            flush_cols<S, KB, ACC, true>(M, pitch, m, n, op, prow, np, pe, pc, pivcol, inv, lq0, lq1, mp);
         else
-            flush_cols<S, KB, ACC, false>(M, pitch, m, n, op, prow, np, pe, inv, lq0, lq1, mp);
+            flush_cols<S, KB, ACC, false>(M, pitch, m, n, op, prow, np, pe, pc, pivcol, inv, lq0, lq1, mp);
     }
     if (op == MODP_ELIM_RREF) {
         // the multipliers have been applied: clear the pivot columns but the pivots
@@ -1042,9 +1188,18 @@ int batched_elim_plan(modp_ctx *ctx, int op, int kind, const modp_elim_job *jobs
     if (e == hipSuccess && count > 0)
         e = hipMemcpyAsync(pl->djobs, hj.data(), sizeof(EJob) * count, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // hj is released on return
-    int act[ECLASS], na = 0;  // classes with work, the largest total cost first
-    for (int c = ECLASS - 1; c >= 0; --c)
-        if (pl->cls[c].count > 0) act[na++] = c;
+    // classes with work, the one with the costliest instance (the longest
+    // single-wavefront latency) first: its workgroups are dispatched before
+    // the other classes fill the CUs' LDS
+    int act[ECLASS], na = 0;
+    int64_t top[ECLASS] = {0, 0, 0};
+    for (int c = 0; c < ECLASS; ++c) {
+        if (pl->cls[c].count == 0) continue;
+        const EJob &f = hj[(size_t)pl->cls[c].off];
+        top[c] = (int64_t)f.m * f.n * (f.m < f.n ? f.m : f.n) * (c == 2 ? 3 : (c == 1 ? 5 : 4));
+        act[na++] = c;
+    }
+    std::sort(act, act + na, [&](int x, int y) { return top[x] > top[y]; });
     for (int i = 0; i < na && e == hipSuccess; ++i)
         e = kind == EK_F64 ? class_launch<double>(pl, act[i], ctx->stream, true)
                            : class_launch<float>(pl, act[i], ctx->stream, true);
@@ -1059,6 +1214,9 @@ int batched_elim_plan(modp_ctx *ctx, int op, int kind, const modp_elim_job *jobs
         k_elim_counter_reset<<<1, 64, 0, ctx->stream>>>(pl->dcounter);
         le = hipGetLastError();
         if (le == hipSuccess && na > 1) le = hipEventRecord(pl->fork, ctx->stream);
+        if (le == hipSuccess && na > 0)
+            le = kind == EK_F64 ? class_launch<double>(pl, act[0], ctx->stream, false)
+                                : class_launch<float>(pl, act[0], ctx->stream, false);
         for (int i = 1; i < na && le == hipSuccess; ++i) {
             hipStream_t st = pl->side[i - 1];
             le = hipStreamWaitEvent(st, pl->fork, 0);
@@ -1067,9 +1225,6 @@ int batched_elim_plan(modp_ctx *ctx, int op, int kind, const modp_elim_job *jobs
                                     : class_launch<float>(pl, act[i], st, false);
             if (le == hipSuccess) le = hipEventRecord(pl->join[i - 1], st);
         }
-        if (le == hipSuccess && na > 0)
-            le = kind == EK_F64 ? class_launch<double>(pl, act[0], ctx->stream, false)
-                                : class_launch<float>(pl, act[0], ctx->stream, false);
         for (int i = 1; i < na && le == hipSuccess; ++i) le = hipStreamWaitEvent(ctx->stream, pl->join[i - 1], 0);
         e = hipStreamEndCapture(ctx->stream, &pl->graph);
         if (e == hipSuccess) e = le;
