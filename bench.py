@@ -387,7 +387,9 @@ def main():
             "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
             "frac": round(achieved / INT8_PEAK_TOPS, 4) if achieved else None,
             "traffic": traffic, "traffic_source": traffic_src,
-            "algorithmic_bytes": int(3 * 8 * n * n),
+            # the core kernel reads r int8 residue planes of A and of B and
+            # writes r residue planes of C once each
+            "algorithmic_bytes": int(3 * r_mod * n * n),
             "measured_peaks": measured,
             "frac_of_measured": round(achieved / m32, 4) if achieved and m32 else None,
             "mfma_issue": {"int8_products": r_mod,

@@ -145,133 +145,21 @@ struct Shape {
     static constexpr int KK = BK / 32;                    // MFMA k-steps per stage
 };
 
-// LDS rows of BK bytes hold CPR 16-byte chunks; chunk q of row `row` sits in
-// slot q ^ f(row) so that 8 consecutive rows read at one chunk (the 8 lanes
-// of one LDS cycle of a fragment read) and one row written by 8 lanes both
-// touch 8 distinct 16-byte bank groups
+// LDS rows of BK = 128 bytes hold 8 16-byte chunks; chunk q of row `row`
+// sits in slot q ^ ((row >> 1) & 7).  The LDS has 64 banks of 4 bytes (256
+// bytes per bank row), so rows r and r + 1 fall in opposite halves of the
+// banks and the 16-byte bank group of (row, slot) is 8 (row & 1) + slot: the
+// 16 consecutive rows of one fragment read at one chunk (16 lanes per LDS
+// cycle) and the 2 rows of 8 lanes of a staging write all land in distinct
+// bank groups (the counters show no bank conflicts; q ^ (row & 7) left
+// rows r and r + 8 on one group).
 template <int CPR> __device__ __forceinline__ int slot(int row, int q)
 {
-    if constexpr (CPR == 8) return q ^ (row & 7);
-    else if constexpr (CPR == 4) return q ^ ((row >> 1) & 3);
-    else return q ^ ((row >> 2) & 1);
+    static_assert(CPR == 8, "128-byte LDS rows");
+    return q ^ ((row >> 1) & 7);
 }
 
-template <class SH>
-__global__ __launch_bounds__(SH::NT, 1) void k_rns_gemm(const int8_t *__restrict__ X, const int8_t *__restrict__ Y,
-                                                      int64_t Np, int64_t Mp, int64_t Kp,
-                                                      uint8_t *__restrict__ E, Moduli md)
-{
-    constexpr int TN = SH::TN, TM = SH::TM, BK = SH::BK, NT = SH::NT, CPR = SH::CPR, KK = SH::KK;
-    constexpr int FN = SH::FN, FM = SH::FM;
-    constexpr int XC = TN * CPR / NT, YC = TM * CPR / NT;  // 16-byte chunks per thread per stage
-    static_assert(TN * CPR % NT == 0 && TM * CPR % NT == 0, "stage chunks must divide among the threads");
-    __shared__ __attribute__((aligned(16))) int8_t Xs[2][TN * BK];
-    __shared__ __attribute__((aligned(16))) int8_t Ys[2][TM * BK];
-
-    int64_t ti, tj;
-    if (!tile_of_block(Np / TN, Mp / TM, &ti, &tj)) return;
-    const int plane = blockIdx.y;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wn = wave % SH::WGN, wm = wave / SH::WGN;
-    const int64_t n0 = ti * TN, m0 = tj * TM;
-    const int8_t *Xg = X + (size_t)plane * Np * Kp + (size_t)n0 * Kp;
-    const int8_t *Yg = Y + (size_t)plane * Mp * Kp + (size_t)m0 * Kp;
-
-    v16i acc[FN][FM];
-#pragma unroll
-    for (int a = 0; a < FN; ++a)
-#pragma unroll
-        for (int b = 0; b < FM; ++b) acc[a][b] = v16i{0};
-
-    v4i rx[XC], ry[YC];
-    auto gload = [&](int64_t k0) {
-#pragma unroll
-        for (int c = 0; c < XC; ++c) {
-            const int idx = tid + NT * c, row = idx / CPR, q = idx % CPR;
-            rx[c] = *reinterpret_cast<const v4i *>(Xg + (size_t)row * Kp + k0 + 16 * q);
-        }
-#pragma unroll
-        for (int c = 0; c < YC; ++c) {
-            const int idx = tid + NT * c, row = idx / CPR, q = idx % CPR;
-            ry[c] = *reinterpret_cast<const v4i *>(Yg + (size_t)row * Kp + k0 + 16 * q);
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int c = 0; c < XC; ++c) {
-            const int idx = tid + NT * c, row = idx / CPR, q = idx % CPR;
-            *reinterpret_cast<v4i *>(&Xs[buf][row * BK + 16 * slot<CPR>(row, q)]) = rx[c];
-        }
-#pragma unroll
-        for (int c = 0; c < YC; ++c) {
-            const int idx = tid + NT * c, row = idx / CPR, q = idx % CPR;
-            *reinterpret_cast<v4i *>(&Ys[buf][row * BK + 16 * slot<CPR>(row, q)]) = ry[c];
-        }
-    };
-    // fragments of k-step kk: lane holds row (lane % 32) of a 32-row block and
-    // the 16 bytes of chunk 2 kk + lane / 32
-    v4i xf[2][FN], yf[2][FM];
-    auto lfrag = [&](int buf, int kk, int set) {
-        const int q = 2 * kk + (lane >> 5);
-#pragma unroll
-        for (int a = 0; a < FN; ++a) {
-            const int row = (wn * FN + a) * 32 + (lane & 31);
-            xf[set][a] = *reinterpret_cast<const v4i *>(&Xs[buf][row * BK + 16 * slot<CPR>(row, q)]);
-        }
-#pragma unroll
-        for (int b = 0; b < FM; ++b) {
-            const int row = (wm * FM + b) * 32 + (lane & 31);
-            yf[set][b] = *reinterpret_cast<const v4i *>(&Ys[buf][row * BK + 16 * slot<CPR>(row, q)]);
-        }
-    };
-
-    const int64_t ns = Kp / BK;
-    gload(0);
-    lstore(0);
-    if (ns > 1) gload(BK);
-    __syncthreads();
-    for (int64_t s = 0; s < ns; ++s) {
-        const int buf = (int)(s & 1);
-        // buffer buf ^ 1 was last read before the previous barrier
-        if (s + 1 < ns) lstore(buf ^ 1);
-        if (s + 2 < ns) gload((s + 2) * BK);
-        lfrag(buf, 0, 0);
-#pragma unroll
-        for (int kk = 0; kk < KK; ++kk) {
-            if (kk + 1 < KK) lfrag(buf, kk + 1, (kk + 1) & 1);
-#pragma unroll
-            for (int a = 0; a < FN; ++a)
-#pragma unroll
-                for (int b = 0; b < FM; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xf[kk & 1][a], yf[kk & 1][b], acc[a][b], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-
-    // accumulator layout of a 32x32 block: lane holds column lane % 32 and
-    // rows 8 (e / 4) + 4 (lane / 32) + e % 4
-    const int q = md.q[plane];
-    const double invq = md.invq[plane];
-    uint8_t *Eg = E + (size_t)plane * Np * Mp;
-#pragma unroll
-    for (int a = 0; a < FN; ++a)
-#pragma unroll
-        for (int b = 0; b < FM; ++b) {
-            const int64_t j = m0 + (wm * FM + b) * 32 + (lane & 31);
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int64_t i = n0 + (wn * FN + a) * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-                const int v = acc[a][b][e];
-                int rr = v - q * (int)floor((double)v * invq);
-                rr += rr < 0 ? q : 0;
-                rr -= rr >= q ? q : 0;
-                Eg[(size_t)i * Mp + j] = (uint8_t)rr;
-            }
-        }
-}
-
-
-// The same core with an explicitly scheduled main loop.  Per thread, every
+// The core with an explicitly scheduled main loop.  Per thread, every
 // global chunk of a stage sits at one 32-bit offset from a workgroup-uniform
 // base (rows 32 apart add a uniform step) and every LDS chunk at one offset
 // plus an immediate; the fragments of k-step kk + 1 are read while the
@@ -308,8 +196,8 @@ __global__ __launch_bounds__(SH::NT, 1) void k_rns_gemm2(const int8_t *__restric
     const int srow = tid / CPR, sq = tid % CPR;
     const uint32_t goff = (uint32_t)(srow * Kp + 16 * sq);
     const uint32_t loff = (uint32_t)(srow * BK + 16 * slot<CPR>(srow, sq));
-    // fragments: rows (w * F + a) * 32 + lane % 32 all share row % 8 = lane % 8
-    const int h = lane >> 5, l7 = lane & 7;
+    // fragments: rows (w * F + a) * 32 + lane % 32 all share (row >> 1) % 8
+    const int h = lane >> 5, l7 = (lane >> 1) & 7;
     uint32_t fx[KK], fy[KK];
 #pragma unroll
     for (int kk = 0; kk < KK; ++kk) {
