@@ -38,16 +38,26 @@ struct BItem {
 
 // Work items are grouped in classes (tile 64 or 32) x (reduction needed or
 // not); each class is served by its own persistent kernel, which carries
-// only that class's tile code (fewer registers, more resident waves), and
-// pulls items from its own counter.
+// only that class's tile code (fewer registers, more resident waves).
+// Within a class every product is given to one of the 8 XCDs (balanced by
+// cost) and its tiles go to that XCD's queue: the workgroups of an XCD (the
+// dispatcher deals workgroups to the XCDs in turn, so workgroup w runs on
+// XCD w mod 8) take their tiles first, so the tiles of one product share
+// its operands in one L2; a workgroup whose queue is empty then helps the
+// others.
 constexpr int NCLASS = 4;
+constexpr int NXCD = 8;
+struct XcdRanges {
+    int64_t off[NXCD + 1];  // item ranges of the XCD queues within a class
+};
 struct modp_gemm_plan {
     int kind = EK_F64;
     int64_t count = 0, items = 0;
     int64_t cls_off[NCLASS + 1] = {0};  // item ranges of the classes
+    XcdRanges xr[NCLASS];               // XCD queues of each class (relative to cls_off)
     BJob *djobs = nullptr;
     BItem *ditems = nullptr;
-    int *dcounter = nullptr;  // one counter per class, 64 bytes apart
+    int *dcounter = nullptr;  // one counter per class and XCD, 64 bytes apart
     int grid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -57,28 +67,34 @@ namespace {
 
 __global__ void k_queue_reset(int *counter)
 {
-    if (threadIdx.x < NCLASS && blockIdx.x == 0) counter[threadIdx.x * 16] = 0;
+    if (threadIdx.x < NCLASS * NXCD && blockIdx.x == 0) counter[threadIdx.x * 16] = 0;
 }
 
 template <typename T, int TILE, int RED>
 __global__ __launch_bounds__(256, 2) void k_batched_gemm(const BJob *__restrict__ jobs,
-                                                         const BItem *__restrict__ items, int64_t nitems,
+                                                         const BItem *__restrict__ items, XcdRanges xr,
                                                          int *__restrict__ counter)
 {
     __shared__ F64Smem<TILE, TILE> sm;
     __shared__ int s_item;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
-        __syncthreads();
-        const int it = s_item;
-        __syncthreads();
-        if (it >= nitems) break;
-        const BItem item = items[it];
-        const BJob &j = jobs[item.job];
-        const int ti = item.tile / j.tiles_n, tj = item.tile % j.tiles_n;
-        gemm_f64_tile<T, T, T, TILE, TILE, EPI_STORE, RED>(j.m, j.n, j.k, (const T *)j.A, j.lda, (const T *)j.B,
-                                                           j.ldb, (T *)j.C, j.ldc, j.mp, 1, (int64_t)ti * TILE,
-                                                           (int64_t)tj * TILE, sm);
+    const int home = blockIdx.x % NXCD;
+    for (int qq = 0; qq < NXCD; ++qq) {
+        const int q = (home + qq) % NXCD;
+        const int64_t base = xr.off[q], cnt = xr.off[q + 1] - base;
+        if (cnt <= 0) continue;
+        for (;;) {
+            if (threadIdx.x == 0) s_item = atomicAdd(counter + 16 * q, 1);
+            __syncthreads();
+            const int it = s_item;
+            __syncthreads();
+            if (it >= cnt) break;
+            const BItem item = items[base + it];
+            const BJob &j = jobs[item.job];
+            const int ti = item.tile / j.tiles_n, tj = item.tile % j.tiles_n;
+            gemm_f64_tile<T, T, T, TILE, TILE, EPI_STORE, RED>(j.m, j.n, j.k, (const T *)j.A, j.lda,
+                                                               (const T *)j.B, j.ldb, (T *)j.C, j.ldc, j.mp, 1,
+                                                               (int64_t)ti * TILE, (int64_t)tj * TILE, sm);
+        }
     }
 }
 
@@ -88,16 +104,20 @@ void launch_class(int cls, const modp_gemm_plan *pl, hipStream_t st)
     const int64_t n = pl->cls_off[cls + 1] - pl->cls_off[cls];
     if (n <= 0) return;
     const BItem *it = pl->ditems + pl->cls_off[cls];
-    int *ctr = pl->dcounter + cls * 16;
+    int *ctr = pl->dcounter + cls * 16 * NXCD;
+    const XcdRanges &xr = pl->xr[cls];
     // resident workgroups per CU by the kernels' register use: 3 for 64x64
     // tiles, 5 for 32x32
     const int64_t cap = (int64_t)pl->grid * (cls < 2 ? 3 : 5);
-    const unsigned g = (unsigned)(n < cap ? n : cap);
+    // a multiple of the XCD count, so every XCD gets the same number
+    int64_t g64 = n < cap ? n : cap;
+    g64 = (g64 + NXCD - 1) / NXCD * NXCD;
+    const unsigned g = (unsigned)g64;
     switch (cls) {
-    case 0: k_batched_gemm<T, 64, RED_NONE><<<g, 256, 0, st>>>(pl->djobs, it, n, ctr); break;
-    case 1: k_batched_gemm<T, 64, RED_STEP><<<g, 256, 0, st>>>(pl->djobs, it, n, ctr); break;
-    case 2: k_batched_gemm<T, 32, RED_NONE><<<g, 256, 0, st>>>(pl->djobs, it, n, ctr); break;
-    default: k_batched_gemm<T, 32, RED_STEP><<<g, 256, 0, st>>>(pl->djobs, it, n, ctr); break;
+    case 0: k_batched_gemm<T, 64, RED_NONE><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    case 1: k_batched_gemm<T, 64, RED_STEP><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    case 2: k_batched_gemm<T, 32, RED_NONE><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    default: k_batched_gemm<T, 32, RED_STEP><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
     }
 }
 
@@ -237,18 +257,34 @@ int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count
     std::vector<BItem> items;
     for (int c = 0; c < NCLASS; ++c) {
         pl->cls_off[c] = (int64_t)items.size();
+        // products to XCDs, largest first to the least loaded XCD
+        std::vector<int64_t> prods;
         for (int64_t oi : order) {
             const BJob &d = hj[oi];
-            if (d.m <= 0 || d.n <= 0 || cls_of(d) != c) continue;
-            const int tiles = ((d.m + d.tile - 1) / d.tile) * d.tiles_n;
-            for (int t = 0; t < tiles; ++t) items.push_back(BItem{(int)oi, t});
+            if (d.m > 0 && d.n > 0 && cls_of(d) == c) prods.push_back(oi);
         }
+        auto tiles_of = [&](const BJob &d) { return ((d.m + d.tile - 1) / d.tile) * d.tiles_n; };
+        std::vector<int64_t> load(NXCD, 0);
+        std::vector<std::vector<int64_t>> per(NXCD);
+        for (int64_t oi : prods) {
+            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[x] += cost(oi) * tiles_of(hj[oi]);
+            per[x].push_back(oi);
+        }
+        for (int x = 0; x < NXCD; ++x) {
+            pl->xr[c].off[x] = (int64_t)items.size() - pl->cls_off[c];
+            for (int64_t oi : per[x]) {
+                const int tiles = tiles_of(hj[oi]);
+                for (int t = 0; t < tiles; ++t) items.push_back(BItem{(int)oi, t});
+            }
+        }
+        pl->xr[c].off[NXCD] = (int64_t)items.size() - pl->cls_off[c];
     }
     pl->cls_off[NCLASS] = (int64_t)items.size();
     pl->items = (int64_t)items.size();
     hipError_t e = hipMalloc(&pl->djobs, sizeof(BJob) * (count > 0 ? count : 1));
     if (e == hipSuccess) e = hipMalloc(&pl->ditems, sizeof(BItem) * (items.empty() ? 1 : items.size()));
-    if (e == hipSuccess) e = hipMalloc(&pl->dcounter, sizeof(int) * 64);
+    if (e == hipSuccess) e = hipMalloc(&pl->dcounter, sizeof(int) * 16 * NCLASS * NXCD);
     if (e == hipSuccess && count > 0)
         e = hipMemcpyAsync(pl->djobs, hj.data(), sizeof(BJob) * count, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && !items.empty())
