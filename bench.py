@@ -1,13 +1,14 @@
-"""Benchmark: mod-p GEMM at n = 8192, p = 2^26 - 5, int8-limb MFMA form vs the
-plain FP64-MFMA form (bit-exact), plus PLUQ seconds at 4096^2 mod 65521 and
-the other workloads of the hot path (1024^2 plain GEMM, p = 127, batched
-products, batched eliminations).
+"""Benchmark: mod-p GEMM at n = 8192, p = 2^26 - 5 in the int8 multi-modular
+MFMA form (the form the library picks there) against the int8 digit form and
+the plain FP64-MFMA form (all bit-exact), plus PLUQ seconds at 4096^2 mod
+65521 and the other workloads of the hot path (1024^2 plain GEMM, p = 127,
+GEMV, batched products, batched eliminations).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--only gemm|pluq|batched]
 
 One "step" is one C = A*B mod p over 8192^3 residues resident in HBM (the
-limb split, the int8 MFMA products and the recombination all inside the
-step).  Prints one JSON line.  The 8192^2 operands are generated on the
+residue splits of A and B, the 9 int8 MFMA products and the Chinese-remainder
+recombination all inside the step).  Prints one JSON line.  The 8192^2 operands are generated on the
 device by the library's seeded generator; the batched workloads come from
 the shared host generator (gpufinitefieldmatrices.workloads) and are
 uploaded once before any timing.

@@ -45,14 +45,6 @@ struct Moduli {
     double wm;                // M mod p
 };
 
-// balanced int8 residue of x (any int32) mod the compile-time modulus Q
-template <int Q> __device__ __forceinline__ int bal_res(int x)
-{
-    int v = x % Q;
-    v += v < 0 ? Q : 0;
-    return v >= (Q + 1) / 2 ? v - Q : v;
-}
-
 // Balanced residue of x (|x| < 2^25) mod the compile-time modulus Q in FP32:
 // with x = xh 4096 + xl (0 <= xl < 4096, |xh| <= 2^13), t = xh (4096 mod Q)
 // + xl is congruent to x and |t| < 2^21, so t, the product t (1/Q) and
