@@ -1,11 +1,11 @@
 // gemm_tile.h -- one BM x BN output tile of the plain FP64-MFMA mod-p GEMM,
 // shared by the single-product kernel and the batched persistent kernel.
 //
-// Operands are reduced on load and mapped to balanced representatives in
-// [-floor(p/2), floor(p/2)]; v_mfma_f64_16x16x4f64 adds four products per
-// step; every R steps the accumulators are brought back into (-p, p) with
-// acc - p*rint(acc/p), so |acc| <= p + R*4*floor(p/2)^2 < 2^53 and every
-// partial sum is an exact integer.
+// Operands are reduced on load to balanced representatives of magnitude at
+// most (p+1)/2 (one rounded quotient each); v_mfma_f64_16x16x4f64 adds four
+// products per step; every R steps the accumulators are brought back into
+// (-p, p) with acc - p*rint(acc/p), so |acc| <= p + R*4*((p+1)/2)^2 <= 2^53
+// and every partial sum is an exact integer.
 #pragma once
 #include "common.h"
 
@@ -37,6 +37,21 @@ static inline int f64_reduce_mode(int R, int64_t k, int *rq)
     return RED_STEP;
 }
 
+// balanced representative of a stored element: for the floating types one
+// rounded quotient, x - p rint(x/p), which for integral |x| < 2^51 lies in
+// [-(p+1)/2, (p+1)/2] (the rounded quotient is within 1/2 + 1/(2p) of x/p);
+// int32 working arrays already hold residues in [0, p)
+template <typename T> __device__ __forceinline__ double bal_of(T x, const ModP &mp)
+{
+    const double d = (double)x;
+    return fma(-rint(d * mp.pinv), mp.pd, d);
+}
+template <> __device__ __forceinline__ double bal_of<int32_t>(int32_t x, const ModP &mp)
+{
+    const double d = (double)x;
+    return d > (double)mp.half ? d - mp.pd : d;
+}
+
 template <typename TA, typename TB, typename TC, int BM, int BN, int EPI, int RED>
 __device__ __forceinline__ void gemm_f64_tile(int64_t m, int64_t n, int64_t k, const TA *__restrict__ A, int64_t lda,
                                               const TB *__restrict__ B, int64_t ldb, TC *__restrict__ C, int64_t ldc,
@@ -59,7 +74,6 @@ __device__ __forceinline__ void gemm_f64_tile(int64_t m, int64_t n, int64_t k, c
         for (int j = 0; j < FN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
     double ra[APT], rb[BPT];
-    const double hp = (double)mp.half;
 
     auto load = [&](int64_t k0) {
 #pragma unroll
@@ -67,24 +81,14 @@ __device__ __forceinline__ void gemm_f64_tile(int64_t m, int64_t n, int64_t k, c
             int idx = tid + 256 * i;
             int row = idx % BM, kk = idx / BM;
             int64_t gr = r0 + row, gk = k0 + kk;
-            double v = 0.0;
-            if (gr < m && gk < k) {
-                double x = (double)to_res<TA>(A[gr + gk * lda], mp);
-                v = x > hp ? x - mp.pd : x;
-            }
-            ra[i] = v;
+            ra[i] = (gr < m && gk < k) ? bal_of<TA>(A[gr + gk * lda], mp) : 0.0;
         }
 #pragma unroll
         for (int i = 0; i < BPT; ++i) {
             int idx = tid + 256 * i;
             int kk = idx % PBK, col = idx / PBK;
             int64_t gk = k0 + kk, gc = c0 + col;
-            double v = 0.0;
-            if (gk < k && gc < n) {
-                double x = (double)to_res<TB>(B[gk + gc * ldb], mp);
-                v = x > hp ? x - mp.pd : x;
-            }
-            rb[i] = v;
+            rb[i] = (gk < k && gc < n) ? bal_of<TB>(B[gk + gc * ldb], mp) : 0.0;
         }
     };
     auto stash = [&](int buf) {
@@ -188,7 +192,8 @@ __device__ __forceinline__ void gemm_f64_tile(int64_t m, int64_t n, int64_t k, c
 // reduction interval: |acc| <= p + R*4*h^2 < 2^53 with h = floor(p/2)
 static inline int f64_reduce_interval(int64_t p)
 {
-    double h = (double)(p / 2);
+    // operands are balanced to at most (p+1)/2 in magnitude (bal_of)
+    double h = (double)((p + 1) / 2);
     double lim = 9007199254740992.0 - (double)p;
     double Rd = h > 0 ? floor(lim / (4.0 * h * h)) : 1e9;
     if (Rd < 1) return 0;
