@@ -6,9 +6,9 @@
 // |C_int| <= k floor(p/2)^2.  That integer is taken modulo r pairwise coprime
 // moduli q_0..q_{r-1} <= 256 with M = prod q_j > 2 |C_int|:
 //
-//   * split: every operand entry is reduced mod each q_j into a balanced
-//     residue in [-128, 127] held in int8 (one plane per modulus); the B
-//     side is multiplied by y_j = (M/q_j)^{-1} mod q_j on the way;
+//   * split: every operand entry is reduced mod each q_j into a residue
+//     held in int8 (one plane per modulus, |residue| <= 128); the B side is
+//     multiplied by y_j = (M/q_j)^{-1} mod q_j on the way;
 //   * core: one int8 GEMM per modulus on v_mfma_i32_32x32x32_i8 with int32
 //     accumulation (|partial sums| <= k 2^14 < 2^31 for k <= 131071), whose
 //     epilogue writes e_j = D_j mod q_j = (y_j C_int) mod q_j as one byte;
@@ -45,46 +45,36 @@ struct Moduli {
     double wm;                // M mod p
 };
 
-// Balanced residue of x (|x| < 2^25) mod the compile-time modulus Q in FP32:
-// with x = xh 4096 + xl (0 <= xl < 4096, |xh| <= 2^13), t = xh (4096 mod Q)
-// + xl is congruent to x and |t| < 2^21, so t, the product t (1/Q) and
-// t - f Q are exact or within one of the integer quotient; one fix-up each
-// way gives t mod Q in [0, Q).  xh and xl come in as floats.
-template <int Q> __device__ __forceinline__ int bal_res_f(float xh, float xl)
-{
-    constexpr float C = (float)(4096 % Q), QF = (float)Q, IQ = 1.0f / (float)Q;
-    const float t = fmaf(xh, C, xl);
-    float r = fmaf(-floorf(t * IQ), QF, t);
-    r += r < 0.0f ? QF : 0.0f;
-    r -= r >= QF ? QF : 0.0f;
-    r -= r >= (float)((Q + 1) / 2) ? QF : 0.0f;
-    return (int)r;
-}
-
-// balanced residue of v y mod Q for |v| <= 128, 0 <= y < 256 (|v y| < 2^15)
-template <int Q> __device__ __forceinline__ int bal_mul_f(int v, int y)
+// A representative of t mod Q in int8 for an integer |t| < 3.2e6 held
+// exactly in a float: r = t - Q rint(t / Q) with the quotient taken from the
+// rounded product t (1/Q).  That product is within 2^-9 of t / Q, and for odd
+// Q the fraction t / Q - n is never closer than 1 / (2Q) > 2^-9 to 1/2, so r
+// is the balanced residue, |r| <= (Q - 1) / 2 <= 127; for Q = 256, r lies in
+// [-128, 128] and the byte cast maps 128 to -128 (the same residue mod 256).
+// The fma is exact (every term is an integer below 2^24).
+template <int Q> __device__ __forceinline__ int res8(float t)
 {
     constexpr float QF = (float)Q, IQ = 1.0f / (float)Q;
-    const float t = (float)(v * y);
-    float r = fmaf(-floorf(t * IQ), QF, t);
-    r += r < 0.0f ? QF : 0.0f;
-    r -= r >= QF ? QF : 0.0f;
-    r -= r >= (float)((Q + 1) / 2) ? QF : 0.0f;
-    return (int)r;
+    return (int)fmaf(-rintf(t * IQ), QF, t);
 }
 
-// the 16 balanced residue bytes of x[0..15] mod Q, packed little-endian;
-// mulY: each residue times y (mod Q) first (the B side)
+// the 16 residue bytes of x[0..15] mod Q, packed little-endian, with
+// x = xh 4096 + xl (|xh| <= 2^13, 0 <= xl < 4096):
+//   A side  t = xh (4096 mod Q) + xl,          |t| < 2^21;
+//   B side  t = xh (4096 y mod Q) + xl y,      |t| < 255 (2^13 + 4096),
+// i.e. x y mod Q in one reduction (y < Q is the CRT weight of the plane).
 template <int Q, bool mulY> __device__ __forceinline__ v4i pack16q(const float *xh, const float *xl, int y)
 {
+    const float C = mulY ? (float)((4096 * y) % Q) : (float)(4096 % Q);
+    const float yf = (float)y;
     v4i w;
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
         uint32_t u = 0;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            int v = bal_res_f<Q>(xh[4 * d + e], xl[4 * d + e]);
-            if (mulY) v = bal_mul_f<Q>(v, y);
+            const float lo = mulY ? xl[4 * d + e] * yf : xl[4 * d + e];
+            const int v = res8<Q>(fmaf(xh[4 * d + e], C, lo));
             u |= ((uint32_t)v & 255u) << (8 * e);
         }
         w[d] = (int)u;
@@ -330,32 +320,60 @@ template <typename T> __device__ __forceinline__ int balanced(T v, const ModP &m
 }
 
 // A (m x k column-major) -> planes Y[j][row][kk] (k contiguous), zero-padded
-// to Mp x Kp; 64x64 tiles transposed through LDS.  blockIdx.x: k tile,
-// blockIdx.y: row tile.
+// to Mp x Kp; 64 x 128 tiles (rows x k) transposed through LDS.  Loads: each
+// lane reads two consecutive rows of one column (16 bytes for FP64), a wave
+// two 512-byte column segments.  Stores: eight lanes write a row's 128 bytes
+// of a plane, a wave eight full rows.  blockIdx.x: k tile, blockIdx.y: row
+// tile.
 template <typename TA>
 __global__ __launch_bounds__(256) void k_split_rows(int64_t m, int64_t k, const TA *__restrict__ A, int64_t lda,
                                                    int8_t *__restrict__ Yp, int64_t Mp, int64_t Kp, ModP mp,
                                                    Moduli md, const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    __shared__ int32_t tile[64][65];
-    const int tid = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 64;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int row = tid & 63, kk = (tid >> 6) + 4 * i;
-        const int64_t gr = r0 + row, gk = k0 + kk;
-        tile[row][kk] = (gr < m && gk < k) ? balanced<TA>(A[gr + gk * lda], mp) : 0;
+    constexpr int TR = 64, TK = 128, LDT = TK + 1;  // odd pitch: conflict-free column writes
+    __shared__ int32_t tile[TR * LDT];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t r0 = (int64_t)blockIdx.y * TR, k0 = (int64_t)blockIdx.x * TK;
+    {
+        const int rr = 2 * (lane & 31);
+        const int64_t gr = r0 + rr;
+        const bool pair = gr + 1 < m && (((uintptr_t)A + (size_t)gr * sizeof(TA)) & (2 * sizeof(TA) - 1)) == 0 &&
+                          (lda & 1) == 0;
+#pragma unroll 4
+        for (int it = 0; it < TK / 8; ++it) {
+            const int kk = 8 * it + 2 * wave + (lane >> 5);
+            const int64_t gk = k0 + kk;
+            int x0 = 0, x1 = 0;
+            if (gk < k) {
+                const TA *src = A + gr + gk * lda;
+                if (pair) {
+                    TA v2[2];
+                    __builtin_memcpy(v2, src, sizeof(v2));
+                    x0 = balanced<TA>(v2[0], mp);
+                    x1 = balanced<TA>(v2[1], mp);
+                } else {
+                    if (gr < m) x0 = balanced<TA>(src[0], mp);
+                    if (gr + 1 < m) x1 = balanced<TA>(src[1], mp);
+                }
+            }
+            tile[rr * LDT + kk] = x0;
+            tile[(rr + 1) * LDT + kk] = x1;
+        }
     }
     __syncthreads();
-    const int row = tid >> 2, ch = tid & 3;
-    float xh[16], xl[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) split12(tile[row][ch * 16 + e], xh[e], xl[e]);
     const size_t plane = (size_t)Mp * Kp;
-    int8_t *dst = Yp + (size_t)(r0 + row) * Kp + k0 + ch * 16;
+    const int ch = tid & 7;
 #pragma unroll 1
-    for (int j = 0; j < md.r; ++j) *reinterpret_cast<v4i *>(dst + j * plane) = pack16<false>(xh, xl, j, md);
+    for (int pass = 0; pass < 2; ++pass) {
+        const int row = 32 * pass + (tid >> 3);
+        float xh[16], xl[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) split12(tile[row * LDT + ch * 16 + e], xh[e], xl[e]);
+        int8_t *dst = Yp + (size_t)(r0 + row) * Kp + k0 + ch * 16;
+#pragma unroll 1
+        for (int j = 0; j < md.r; ++j) *reinterpret_cast<v4i *>(dst + j * plane) = pack16<false>(xh, xl, j, md);
+    }
 }
 
 // B (k x n column-major) -> planes X[j][col][kk] (k contiguous, times y_j),
@@ -372,10 +390,22 @@ __global__ __launch_bounds__(256) void k_split_cols(int64_t k, int64_t n, const 
     if (e >= Np * chunks) return;
     const int64_t col = e / chunks, k0 = (e % chunks) * 16;
     float xh[16], xl[16];
+    const TB *src = B + k0 + col * ldb;
+    if (col < n && k0 + 16 <= k && ((uintptr_t)src & 15) == 0) {
+        // the whole chunk in range and 16-byte aligned: sizeof(TB) 16-byte loads
+        v4i raw[sizeof(TB)];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        const int64_t gk = k0 + t;
-        split12((col < n && gk < k) ? balanced<TB>(B[gk + col * ldb], mp) : 0, xh[t], xl[t]);
+        for (int i = 0; i < (int)sizeof(TB); ++i) raw[i] = reinterpret_cast<const v4i *>(src)[i];
+        TB v[16];
+        __builtin_memcpy(v, raw, sizeof(v));
+#pragma unroll
+        for (int t = 0; t < 16; ++t) split12(balanced<TB>(v[t], mp), xh[t], xl[t]);
+    } else {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const int64_t gk = k0 + t;
+            split12((col < n && gk < k) ? balanced<TB>(src[t], mp) : 0, xh[t], xl[t]);
+        }
     }
     const size_t plane = (size_t)Np * Kp;
     int8_t *dst = Xp + (size_t)col * Kp + k0;

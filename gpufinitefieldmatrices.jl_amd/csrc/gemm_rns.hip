@@ -74,7 +74,7 @@ template <typename T>
 int split_rows_t(modp_ctx *ctx, const GemmArgs &g, int64_t kbeg, int64_t kc, int8_t *Yp, int64_t Mp, int64_t Kp,
                  const Moduli &md)
 {
-    const dim3 grid((unsigned)(Kp / 64), (unsigned)(Mp / 64));
+    const dim3 grid((unsigned)(Kp / 128), (unsigned)(Mp / 64));
     const T *A = (const T *)g.A + kbeg * g.lda;
     rns::k_split_rows<T><<<grid, 256, 0, ctx->stream>>>(g.m, kc, A, g.lda, Yp, Mp, Kp, g.mp, md, g.skip);
     return hip_check(ctx, hipGetLastError(), "k_split_rows launch");
