@@ -224,7 +224,7 @@ __global__ __launch_bounds__(SH::NT, 1) void k_rns_gemm2(const int8_t *__restric
         for (int a = 0; a < FN; ++a)
 #pragma unroll
             for (int b = 0; b < FM; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xf[set][a], yf[set][b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(yf[set][b], xf[set][a], acc[a][b], 0, 0, 0);
     };
     constexpr int NMF = FN * FM, NFR = FN + FM, NST = (XC + YC) / 2, NGL = (XC + YC) / 2;
     // one k-step: its MFMAs interleaved with the fragment reads of the next
@@ -284,24 +284,51 @@ __global__ __launch_bounds__(SH::NT, 1) void k_rns_gemm2(const int8_t *__restric
         for (int kk = 0; kk < KK; ++kk) step(buf, kk, false, 0, 0, false, 0, 0);
     }
 
+    // Epilogue.  The blocks are D^T (Y rows first), so lane l holds E row
+    // i = column l % 32 of the block and the four consecutive j = 8 g + 4 (l / 32)
+    // + 0..3 of group g, packed into one word.  v mod q in FP32: v = a 2^16 + b
+    // (|a| <= 2^15, 0 <= b < 2^16), t = a (2^16 mod q) + b is exact and below
+    // 2^23, rint(t / q) is within 0.01 of t / q, so r = t - q rint(t / q) is
+    // exact with |r| < q and one fix-up gives v mod q in [0, q).
+    // The words go through LDS (the stage buffer that the last stage does not
+    // read: the barrier closing stage ns - 2 released it) so that every store
+    // writes 16 whole 64-byte row segments: the wave's WTN x WTM bytes sit in
+    // rows of WTM bytes whose 16-byte chunk c is stored at c ^ ((row / 4) % 4)
+    // (the word writes of 32 rows spread over 32 banks, the 16-byte reads of 4
+    // consecutive rows cover all 64).
+    constexpr int WTN = SH::WTN, WTM = SH::WTM, NW = NT / 64;
+    static_assert(WTM == 64 && NW % 2 == 0 && (NW / 2) * WTN * WTM <= TN * BK && TN == TM, "epilogue staging");
     const int q = md.q[plane];
-    const double invq = md.invq[plane];
-    uint8_t *Eg = E + (size_t)plane * Np * Mp;
+    const float qf = (float)q, iqf = 1.0f / qf, c16 = (float)(65536 % q);
+    const int nb = (int)(ns & 1);
+    int8_t *stg = (wave < NW / 2 ? Xs[nb] : Ys[nb]) + (wave % (NW / 2)) * (WTN * WTM);
 #pragma unroll
-    for (int a = 0; a < FN; ++a)
+    for (int a = 0; a < FN; ++a) {
+        const int il = a * 32 + (lane & 31);
 #pragma unroll
-        for (int b = 0; b < FM; ++b) {
-            const int64_t j = m0 + (wm * FM + b) * 32 + (lane & 31);
+        for (int b = 0; b < FM; ++b)
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int64_t i = n0 + (wn * FN + a) * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-                const int v = acc[a][b][e];
-                int rr = v - q * (int)floor((double)v * invq);
-                rr += rr < 0 ? q : 0;
-                rr -= rr >= q ? q : 0;
-                Eg[(size_t)i * Mp + j] = (uint8_t)rr;
+            for (int g = 0; g < 4; ++g) {
+                uint32_t u = 0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int v = acc[a][b][4 * g + t];
+                    const float tv = fmaf((float)(v >> 16), c16, (float)(v & 0xFFFF));
+                    int r = (int)fmaf(-rintf(tv * iqf), qf, tv);
+                    r += r < 0 ? q : 0;
+                    u |= (uint32_t)r << (8 * t);
+                }
+                const int chunk = 2 * b + (g >> 1), w = 2 * (g & 1) + (lane >> 5);
+                *reinterpret_cast<uint32_t *>(stg + il * WTM + 16 * (chunk ^ ((il >> 2) & 3)) + 4 * w) = u;
             }
-        }
+    }
+    uint8_t *Eg = E + (size_t)plane * Np * Mp + (size_t)(n0 + wn * WTN) * Mp + m0 + wm * WTM;
+#pragma unroll
+    for (int it = 0; it < WTN / 16; ++it) {
+        const int row = it * 16 + (lane >> 2), c = lane & 3;
+        const v4i w4 = *reinterpret_cast<const v4i *>(stg + row * WTM + 16 * (c ^ ((row >> 2) & 3)));
+        *reinterpret_cast<v4i *>(Eg + (size_t)row * Mp + 16 * c) = w4;
+    }
 }
 
 }  // namespace rns
