@@ -242,7 +242,7 @@ struct TuneEntry {
     double ms = 0.0;    // measured time of the winner
 };
 
-constexpr int WS_COUNT = 20;
+constexpr int WS_COUNT = 21;
 
 struct modp_ctx {
     int device = 0;
@@ -301,7 +301,8 @@ enum {
     WS_TRI_X2 = 16,    //   second inverse (matrix inverse: L^{-1})
     WS_INV_U = 17,     // matrix inverse: PLUQ factors
     WS_INV_L = 18,
-    WS_PERM = 19       // lines moved by a permutation
+    WS_PERM = 19,      // lines moved by a permutation
+    WS_GPOS = 20       // row positions of the W-resident elimination panel
 };
 
 // error plumbing: every internal function returns a modp_status

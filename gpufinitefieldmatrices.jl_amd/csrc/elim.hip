@@ -18,7 +18,8 @@
 //                extended Euclid), and the rank-1 update of the remaining
 //                panel columns; rows move only at the end, by a row map, and
 //                the old values of the moved rows in the block's other
-//                columns are staged.
+//                columns are staged.  Above 32 x 512 rows k_panel_g does the
+//                same with the panel left in W (any row count).
 //     k_update   the block's other columns: the row map, and right of the
 //                panel the triangular solve with L11 (rows of U) and the
 //                rank-w update of every row below the pivots.
@@ -462,6 +463,218 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
         ctl->status = status;
         ctl->ncols = consumed;
         ctl->nmap = pc + s_cnt;
+        ctl->kr0 = kr0;
+        if (cursor) *cursor = kr0 + pc;
+        if (abort && status != 0) *abort = 1;
+    }
+}
+
+// Panel factorisation for any number of rows (the register panel holds at
+// most 32 x 512 rows and its small-prime key 2^15 positions): the same
+// outputs as k_panel from one 1024-thread workgroup that keeps the panel in
+// W.  Thread t owns local rows t + 1024 q and is the only thread that reads
+// or writes them during the column steps; a row's current position (the
+// reference's swap of positions pc and Ip) is kept in gpos[row], the row at
+// each position in grow[pos].  Per column: the owners scan their eligible
+// rows (position >= pc) with 64-bit keys (residue, then lowest position),
+// the wave winners publish their rows and inverses in LDS, and after one
+// barrier every thread knows the pivot; thread 0 swaps the positions, the
+// lanes below w normalise the pivot row, and after a second barrier the
+// owners apply the rank-1 update to their rows (residues reduced every
+// time).  At the end the moved rows (at most 2 w) go through LDS to their
+// positions, the pivot rows with their 1 and normalised U row.
+constexpr int GPT = 1024, GNBW = 16;
+template <int NBW>
+__global__ __launch_bounds__(GPT) void k_panel_g(int32_t *__restrict__ W, int64_t ldw, int m, int kr0_in, int c0,
+                                                 int w, ModP mp, int ple, const uint32_t *__restrict__ invtab,
+                                                 int32_t *__restrict__ ipiv, int32_t *__restrict__ pivval,
+                                                 int32_t *__restrict__ pivinv, int32_t *__restrict__ pivcol,
+                                                 int32_t *__restrict__ colmap, int32_t *__restrict__ map_dst,
+                                                 int32_t *__restrict__ map_src, PanelCtl *__restrict__ ctl,
+                                                 int *__restrict__ abort, int KC, int FAR, int32_t *__restrict__ stage,
+                                                 int *__restrict__ cursor, int32_t *__restrict__ gpos,
+                                                 int32_t *__restrict__ grow)
+{
+    constexpr int NW = GPT / 64;
+    if (abort && *abort) {
+        if (threadIdx.x == 0) {
+            ctl->npiv = 0;
+            ctl->status = 4;
+            ctl->ncols = 0;
+            ctl->nmap = 0;
+            ctl->kr0 = kr0_in;
+        }
+        return;
+    }
+    const int kr0 = cursor ? *cursor : kr0_in;
+    __shared__ uint64_t s_key[NW];
+    __shared__ int s_row[NW];
+    __shared__ uint32_t s_cinv[NW];
+    __shared__ uint32_t s_crow[NW][NBW];
+    __shared__ uint32_t s_U[NBW][NBW];
+    __shared__ int s_ip[NBW], s_pv[NBW], s_pi[NBW], s_pcol[NBW], s_cmap[NBW];
+    __shared__ int s_msrc[2 * NBW], s_mdst[2 * NBW];
+    __shared__ uint32_t s_mv[2 * NBW][NBW];
+    __shared__ int s_cnt;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rows = m - kr0;
+    int32_t *P = W + kr0 + (int64_t)c0 * ldw;  // local row i, panel column j: P[i + j ldw]
+    for (int i = tid; i < rows; i += GPT) {
+        gpos[i] = i;
+        grow[i] = i;
+    }
+    if (tid == 0) s_cnt = 0;
+    __syncthreads();
+
+    int pc = 0, status = 0, consumed = w;
+    bool active = true;
+    for (int jc = 0; jc < NBW; ++jc) {
+        const bool run = active && jc < w && pc < rows;
+        if (active && jc < w && pc >= rows) {
+            status = 2;
+            consumed = jc;
+            active = false;
+        }
+        uint64_t mine = 0;
+        int mrow = -1;
+        if (run) {
+            for (int i = tid; i < rows; i += GPT) {
+                const int ps = gpos[i];
+                if (ps < pc) continue;
+                const uint32_t v = (uint32_t)P[i + (int64_t)jc * ldw];
+                const uint64_t k = ((uint64_t)v << 32) | (uint32_t)(0x7fffffff - ps);
+                if (v != 0 && k > mine) {
+                    mine = k;
+                    mrow = i;
+                }
+            }
+        }
+        const uint64_t wbest = wave_max(mine);
+        if (mine != 0 && mine == wbest) {
+            // keys are unique: one lane per wave publishes its row
+            const uint32_t V = (uint32_t)(mine >> 32);
+            s_key[wave] = mine;
+            s_row[wave] = mrow;
+            s_cinv[wave] = invtab ? invtab[V] : modinv_fast(V, mp);
+            for (int j = jc + 1; j < NBW; ++j) s_crow[wave][j] = j < w ? (uint32_t)P[mrow + (int64_t)j * ldw] : 0u;
+        }
+        if (lane == 0 && wbest == 0) s_key[wave] = 0;
+        __syncthreads();
+        uint64_t best = s_key[0];
+        int bw = 0;
+        for (int x = 1; x < NW; ++x) {
+            const uint64_t k = s_key[x];
+            if (k > best) {
+                best = k;
+                bw = x;
+            }
+        }
+        const uint32_t V = (uint32_t)(best >> 32);
+        const int Ip = 0x7fffffff - (int)(uint32_t)best;
+        const uint32_t inv = V != 0 ? s_cinv[bw] : 0u;
+        const int wrow = V != 0 ? s_row[bw] : -1;
+        const bool piv = run && V != 0 && inv != 0;
+        if (run && V == 0) {
+            if (ple) {
+                if (tid == 0) s_cmap[jc] = -1;
+            } else {
+                status = 1;
+                consumed = jc;
+                active = false;
+            }
+        }
+        if (run && V != 0 && inv == 0) {
+            status = 3;
+            consumed = jc;
+            active = false;
+        }
+        if (piv) {
+            if (tid == 0) {
+                // the winner moves to position pc, the row at pc to Ip
+                const int rpc = grow[pc];
+                gpos[wrow] = pc;
+                gpos[rpc] = Ip;
+                grow[Ip] = rpc;
+                grow[pc] = wrow;
+                s_ip[pc] = Ip;
+                s_pv[pc] = (int)V;
+                s_pi[pc] = (int)inv;
+                s_pcol[pc] = jc;
+                s_cmap[jc] = pc;
+            }
+            if (tid > jc && tid < NBW) s_U[pc][tid] = tid < w ? mulmod(s_crow[bw][tid], inv, mp) : 0u;
+        }
+        __syncthreads();
+        if (piv) {
+            const int pn = pc + 1;
+            for (int i = tid; i < rows; i += GPT) {
+                if (gpos[i] < pn) continue;
+                const uint32_t l = (uint32_t)P[i + (int64_t)jc * ldw];
+                if (l == 0) continue;
+                for (int j = jc + 1; j < w; ++j) {
+                    int32_t *x = P + i + (int64_t)j * ldw;
+                    *x = (int32_t)submod((uint32_t)*x, mulmod(l, s_U[pc][j], mp), mp);
+                }
+            }
+            pc = pn;
+        }
+        // the next step's LDS writes come after every read of this one
+        __syncthreads();
+    }
+
+    // Row gather map: positions 0..pc-1 first, in order, then every other
+    // row that moved.
+    for (int i = tid; i < rows; i += GPT) {
+        const int ps = gpos[i];
+        if (ps < pc) {
+            map_dst[ps] = kr0 + ps;
+            map_src[ps] = kr0 + i;
+            s_msrc[ps] = i;
+            s_mdst[ps] = ps;
+        } else if (ps != i) {
+            const int e = pc + atomicAdd(&s_cnt, 1);
+            map_dst[e] = kr0 + ps;
+            map_src[e] = kr0 + i;
+            s_msrc[e] = i;
+            s_mdst[e] = ps;
+        }
+    }
+    __syncthreads();
+    const int nmap_all = pc + s_cnt;
+    // old values of the moved rows in the block's other columns
+    const int ncol_other = (FAR - KC) - w, nstage = nmap_all * ncol_other;
+    for (int idx = tid; idx < nstage; idx += GPT) {
+        const int e = idx % nmap_all, ci = idx / nmap_all;
+        const int c = ci < c0 - KC ? KC + ci : KC + ci + w;
+        stage[e + 64 * ci] = W[(int64_t)kr0 + s_msrc[e] + (int64_t)c * ldw];
+    }
+    // the moved rows' final panel values, then their stores
+    for (int idx = tid; idx < nmap_all * NBW; idx += GPT) {
+        const int e = idx % nmap_all, j = idx / nmap_all;
+        if (j < w) {
+            const int k = s_mdst[e];
+            const int jk = k < pc ? s_pcol[k] : NBW;
+            s_mv[e][j] = j < jk ? (uint32_t)P[s_msrc[e] + (int64_t)j * ldw] : (j == jk ? 1u : s_U[k][j]);
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nmap_all * NBW; idx += GPT) {
+        const int e = idx % nmap_all, j = idx / nmap_all;
+        if (j < w) P[s_mdst[e] + (int64_t)j * ldw] = (int32_t)s_mv[e][j];
+    }
+    if (tid < pc) {
+        ipiv[kr0 + tid] = kr0 + s_ip[tid];
+        pivval[kr0 + tid] = s_pv[tid];
+        pivinv[kr0 + tid] = s_pi[tid];
+        pivcol[kr0 + tid] = c0 + s_pcol[tid];
+    }
+    if (tid < w) colmap[tid] = tid < consumed ? s_cmap[tid] : -1;
+    if (tid == 0) {
+        ctl->npiv = pc;
+        ctl->status = status;
+        ctl->ncols = consumed;
+        ctl->nmap = nmap_all;
         ctl->kr0 = kr0;
         if (cursor) *cursor = kr0 + pc;
         if (abort && status != 0) *abort = 1;
@@ -925,8 +1138,8 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
         if (U && m > 0 && n > 0) return MODP_OK;
         return MODP_OK;
     }
-    if (m > 16384)
-        return set_error(ctx, MODP_ERR_DIM, "elimination supports at most 16384 rows (got %lld)", (long long)m);
+    if (m > (int64_t)1 << 30)
+        return set_error(ctx, MODP_ERR_DIM, "elimination supports at most 2^30 rows (got %lld)", (long long)m);
 
     // Panel shape of an outer block starting at row KR: the fewest rows per
     // thread that hold every row from KR's 4-aligned row down (the panel
@@ -936,8 +1149,18 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
     // fewer rows, so their panels are shorter and wider.
     const char *wenv = getenv("MODP_PANEL_W");
     const int want_w = wenv ? atoi(wenv) : 0;
+    // MODP_PANEL_GLOBAL=1 sends every panel to the W-resident kernel (used
+    // to compare the two panel kernels at sizes both handle)
+    const char *genv = getenv("MODP_PANEL_GLOBAL");
+    const bool force_global = genv && atoi(genv) != 0;
     auto panel_shape = [&](int64_t KR, int *RPT, int *NBW) {
         const int64_t need = m - (KR & ~(int64_t)3);
+        if (force_global || need > (int64_t)32 * PT) {
+            // more rows than the register panel holds: W-resident panel
+            *RPT = 0;
+            *NBW = GNBW;
+            return;
+        }
         int r = 1;
         while ((int64_t)r * PT < need) r *= 2;
         int wd = r == 1 ? 32 : (r == 2 ? 16 : (r <= 8 ? 8 : 2));
@@ -968,6 +1191,14 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
     int *cursor = flag + 2;                   // first row of the next panel (lu, rref)
     int32_t *Ubig = (int32_t *)ws_get(ctx, WS_UBUF, sizeof(int32_t) * (size_t)NBO * (size_t)(n + 1), &st);
     if (!Ubig) return st;
+    // row positions of the W-resident panel (position of each row, row at
+    // each position)
+    int32_t *gpos = nullptr, *grow = nullptr;
+    if (force_global || m > (int64_t)32 * PT) {
+        gpos = (int32_t *)ws_get(ctx, WS_GPOS, sizeof(int32_t) * 2 * (size_t)m, &st);
+        if (!gpos) return st;
+        grow = gpos + m;
+    }
     PanelCtl *hctl = (PanelCtl *)pinned_get(ctx, sizeof(PanelCtl) * (MAXPAN + 1) + 64, &st);
     if (!hctl) return st;
     int *abort_flag = flag + 1;
@@ -1002,6 +1233,13 @@ int elim_run(modp_ctx *ctx, int mode, int form, int kind, int64_t m, int64_t n, 
         if (t >= MAXPAN) return set_error(ctx, MODP_ERR_DEVICE, "more than %d panels in an outer block", MAXPAN);
         int32_t *mdst = bmap_dst + t * 64, *msrc = bmap_src + t * 64;
         PanelCtl *pctl = bctl + t;
+        if (RPT == 0) {
+            TimedLaunch tl(ctx, "elim_panel");
+            k_panel_g<GNBW><<<1, GPT, 0, ctx->stream>>>(W, ldw, (int)m, (int)rj, (int)cj, (int)w, mp, ple, invtab, ipiv,
+                                                        pivval, pivinv, pivcol, colmap, mdst, msrc, pctl, abort,
+                                                        (int)KC, (int)FAR, stage, cursor, gpos, grow);
+            return hip_check(ctx, hipGetLastError(), "k_panel_g launch");
+        }
         const int key = RPT * 100 + NBW;
 #define LAUNCH_PANEL(R_, W_) \
     case R_ * 100 + W_: \
