@@ -10,11 +10,14 @@
 // in LDS, the same pivot rule as the single-matrix path; a plan uploads the
 // job table once and captures the persistent kernel in a hipGraph.
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <map>
 #include <numeric>
 #include <type_traits>
 
 #include "batched.h"
+#include "gemm_limb.h"
 #include "gemm_tile.h"
 
 // ---------------------------------------------------------------------------
@@ -29,6 +32,9 @@ struct BJob {
     int tile;     // 32 or 64
     int tiles_n;  // tiles along n
     ModP mp;
+    int L;        // digit form: digits per residue (0: FP64 form)
+    int lazy;     // digit form: the recombination needs no reduction per term
+    double w[7];  // digit form: 256^s mod p
 };
 
 struct BItem {
@@ -36,16 +42,17 @@ struct BItem {
     int tile;  // tile index within the job: ti = tile / tiles_n, tj = tile % tiles_n
 };
 
-// Work items are grouped in classes (tile 64 or 32) x (reduction needed or
-// not); each class is served by its own persistent kernel, which carries
-// only that class's tile code (fewer registers, more resident waves).
+// Work items are grouped in classes: FP64 form (tile 64 or 32) x (reduction
+// needed or not), and the int8 digit form by digit count (1..4); each class
+// is served by its own persistent kernel, which carries only that class's
+// tile code (fewer registers, more resident waves).
 // Within a class every product is given to one of the 8 XCDs (balanced by
 // cost) and its tiles go to that XCD's queue: the workgroups of an XCD (the
 // dispatcher deals workgroups to the XCDs in turn, so workgroup w runs on
 // XCD w mod 8) take their tiles first, so the tiles of one product share
 // its operands in one L2; a workgroup whose queue is empty then helps the
 // others.
-constexpr int NCLASS = 4;
+constexpr int NCLASS = 8;  // 0-3 FP64 form, 4-7 digit form with 1-4 digits
 constexpr int NXCD = 8;
 struct XcdRanges {
     int64_t off[NXCD + 1];  // item ranges of the XCD queues within a class
@@ -98,6 +105,151 @@ __global__ __launch_bounds__(256, 2) void k_batched_gemm(const BJob *__restrict_
     }
 }
 
+// Digit form of one work item: a 64 x 64 output tile on four waves (32 x 32
+// blocks, v_mfma_i32_32x32x32_i8).  Per 64-deep stage every thread loads 16
+// elements of A (one row, the lanes along the rows: 512-byte runs) and 16 of
+// B (lane pairs of consecutive k, the lanes along k: two 512-byte column runs
+// per load), turns each residue into L signed base-256 digits (the limb
+// form's representation) and writes them to the digit planes in LDS; the
+// next stage's loads are in flight during the MFMAs.  The L x L digit
+// products accumulate by digit sum s = u + v in 2L - 1 int32 accumulators
+// (|sum| <= L k 2^14 < 2^31 for k <= 16384), and the epilogue forms
+// sum_s (acc_s mod p) (256^s mod p) mod p in FP64.  The blocks are computed
+// transposed (B digits as the first operand), so lane l holds row l % 32 of
+// C and a store writes 32 consecutive rows of a column.  LDS rows hold 64
+// digits (4 chunks); chunk c of row r sits at slot c ^ ((r >> 2) & 3), which
+// keeps the 16-lane fragment reads free of bank conflicts.
+constexpr int BL_TILE = 64, BL_BK = 64, BL_KMAX = 16384;
+template <typename T, int L>
+__global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict__ jobs,
+                                                         const BItem *__restrict__ items, XcdRanges xr,
+                                                         int *__restrict__ counter)
+{
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    typedef int v16i __attribute__((ext_vector_type(16)));
+    constexpr int TL = BL_TILE, BK = BL_BK, NA = 2 * L - 1;
+    constexpr int32_t TOP = (int32_t)(127 * ((((int64_t)1) << (8 * L)) - 1) / 255);
+    __shared__ __attribute__((aligned(16))) int8_t As[2][L][TL * BK];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[2][L][TL * BK];
+    __shared__ int s_item;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wc = wave >> 1;
+    // A staging: row sr, k chunk sc (16 k); B staging: column 16 wave + 2 i +
+    // lane / 32 (i < 8), k pair 2 (lane % 32)
+    const int sr = tid & 63, sc = tid >> 6;
+    const int aslot = sr * BK + 16 * (sc ^ ((sr >> 2) & 3));
+    const int bk2 = 2 * (lane & 31);
+    const int fr = lane & 31, h = lane >> 5, fsw = (fr >> 2) & 3;
+    const int home = blockIdx.x % NXCD;
+    for (int qq = 0; qq < NXCD; ++qq) {
+        const int q = (home + qq) % NXCD;
+        const int64_t base = xr.off[q], cnt = xr.off[q + 1] - base;
+        if (cnt <= 0) continue;
+        for (;;) {
+            if (threadIdx.x == 0) s_item = atomicAdd(counter + 16 * q, 1);
+            __syncthreads();
+            const int it = s_item;
+            __syncthreads();
+            if (it >= cnt) break;
+            const BItem item = items[base + it];
+            const BJob &j = jobs[item.job];
+            const int ti = item.tile / j.tiles_n, tj = item.tile % j.tiles_n;
+            const int64_t r0 = (int64_t)ti * TL, c0 = (int64_t)tj * TL;
+            const T *A = (const T *)j.A, *B = (const T *)j.B;
+            const ModP mp = j.mp;
+            const int K = j.k, ns = (K + BK - 1) / BK;
+            const int64_t arow = r0 + sr;
+            const bool arow_ok = arow < j.m;
+            T va[16], vb[16];
+            auto gload = [&](int st) {
+                const int64_t kb = (int64_t)st * BK;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int64_t kk = kb + 16 * sc + e;
+                    va[e] = (arow_ok && kk < K) ? A[arow + kk * j.lda] : (T)0;
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int64_t col = c0 + 16 * wave + 2 * i + h;
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        const int64_t kk = kb + bk2 + t;
+                        vb[2 * i + t] = (col < j.n && kk < K) ? B[kk + col * j.ldb] : (T)0;
+                    }
+                }
+            };
+            auto lstore = [&](int buf) {
+                v4i wa[L];
+#pragma unroll
+                for (int u = 0; u < L; ++u) wa[u] = v4i{0, 0, 0, 0};
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    int32_t xa = digit_rep(to_res<T>(va[e], mp), TOP, mp);
+#pragma unroll
+                    for (int u = 0; u < L; ++u)
+                        wa[u][e >> 2] |= (int)(((uint32_t)next_digit(xa) & 255u) << (8 * (e & 3)));
+                }
+#pragma unroll
+                for (int u = 0; u < L; ++u) *reinterpret_cast<v4i *>(&As[buf][u][aslot]) = wa[u];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int c = 16 * wave + 2 * i + h;
+                    const int off = c * BK + 16 * ((bk2 >> 4) ^ ((c >> 2) & 3)) + (bk2 & 15);
+                    int32_t x0 = digit_rep(to_res<T>(vb[2 * i], mp), TOP, mp);
+                    int32_t x1 = digit_rep(to_res<T>(vb[2 * i + 1], mp), TOP, mp);
+#pragma unroll
+                    for (int u = 0; u < L; ++u) {
+                        const uint32_t d0 = (uint32_t)next_digit(x0) & 255u, d1 = (uint32_t)next_digit(x1) & 255u;
+                        *reinterpret_cast<uint16_t *>(&Bs[buf][u][off]) = (uint16_t)(d0 | (d1 << 8));
+                    }
+                }
+            };
+            v16i acc[NA];
+#pragma unroll
+            for (int t = 0; t < NA; ++t) acc[t] = v16i{0};
+            gload(0);
+            lstore(0);
+            __syncthreads();
+            for (int st = 0; st < ns; ++st) {
+                const int buf = st & 1;
+                if (st + 1 < ns) gload(st + 1);
+#pragma unroll
+                for (int kk = 0; kk < BK / 32; ++kk) {
+                    const int off = 16 * ((2 * kk + h) ^ fsw);
+                    v4i fa[L], fb[L];
+#pragma unroll
+                    for (int u = 0; u < L; ++u) {
+                        fa[u] = *reinterpret_cast<const v4i *>(&As[buf][u][(wr * 32 + fr) * BK + off]);
+                        fb[u] = *reinterpret_cast<const v4i *>(&Bs[buf][u][(wc * 32 + fr) * BK + off]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < L; ++u)
+#pragma unroll
+                        for (int v = 0; v < L; ++v)
+                            acc[u + v] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[v], fa[u], acc[u + v], 0, 0, 0);
+                }
+                if (st + 1 < ns) lstore(buf ^ 1);
+                __syncthreads();
+            }
+            const int64_t i = r0 + wr * 32 + fr;
+            T *C = (T *)j.C;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int64_t jj = c0 + wc * 32 + 8 * (e >> 2) + 4 * h + (e & 3);
+                double S = 0.0;
+                if (j.lazy) {
+#pragma unroll
+                    for (int t = 0; t < NA; ++t) S = fma((double)acc[t][e], j.w[t], S);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < NA; ++t) S = dmod(fma(dmod((double)acc[t][e], mp), j.w[t], S), mp);
+                }
+                if (i < j.m && jj < j.n) C[i + jj * j.ldc] = from_res<T>((uint32_t)dmod(S, mp));
+            }
+        }
+    }
+}
+
 template <typename T>
 void launch_class(int cls, const modp_gemm_plan *pl, hipStream_t st)
 {
@@ -106,9 +258,9 @@ void launch_class(int cls, const modp_gemm_plan *pl, hipStream_t st)
     const BItem *it = pl->ditems + pl->cls_off[cls];
     int *ctr = pl->dcounter + cls * 16 * NXCD;
     const XcdRanges &xr = pl->xr[cls];
-    // resident workgroups per CU by the kernels' register use: 3 for 64x64
-    // tiles, 5 for 32x32
-    const int64_t cap = (int64_t)pl->grid * (cls < 2 ? 3 : 5);
+    // resident workgroups per CU by the kernels' register and LDS use: 3
+    // for 64x64 FP64 tiles, 5 for 32x32, 2 for the digit form
+    const int64_t cap = (int64_t)pl->grid * (cls >= 4 ? 2 : (cls < 2 ? 3 : 5));
     // a multiple of the XCD count, so every XCD gets the same number
     int64_t g64 = n < cap ? n : cap;
     g64 = (g64 + NXCD - 1) / NXCD * NXCD;
@@ -117,7 +269,11 @@ void launch_class(int cls, const modp_gemm_plan *pl, hipStream_t st)
     case 0: k_batched_gemm<T, 64, RED_NONE><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
     case 1: k_batched_gemm<T, 64, RED_STEP><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
     case 2: k_batched_gemm<T, 32, RED_NONE><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
-    default: k_batched_gemm<T, 32, RED_STEP><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    case 3: k_batched_gemm<T, 32, RED_STEP><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    case 4: k_batched_limb<T, 1><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    case 5: k_batched_limb<T, 2><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    case 6: k_batched_limb<T, 3><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    default: k_batched_limb<T, 4><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
     }
 }
 
@@ -231,6 +387,10 @@ namespace {
 
 int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count, double thr, modp_gemm_plan **out)
 {
+    // MODP_BATCHED_FP64=1 keeps every product in the FP64 form (used to
+    // compare the two forms)
+    const char *fenv = getenv("MODP_BATCHED_FP64");
+    const bool fp64_only = fenv && atoi(fenv) != 0;
     modp_gemm_plan *pl = new modp_gemm_plan();
     pl->kind = kind;
     pl->count = count;
@@ -245,12 +405,29 @@ int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count
         d.R = f64_reduce_interval(s.p);
         d.mp = make_modp(s.p);
         d.tile = pick_tile(d.m, d.n, thr);
+        d.L = 0;
+        d.lazy = 0;
+        const int L = limb_count(s.p);
+        if (!fp64_only && d.k <= BL_KMAX && L <= 4) {
+            // digit form: exact when every term of the recombination,
+            // |acc_s| (256^s mod p) with |acc_s| <= L k 2^14, sums below 2^53
+            d.L = L;
+            d.tile = BL_TILE;
+            double w = 1.0;
+            for (int t = 0; t < 7; ++t) {
+                d.w[t] = w;
+                w = std::fmod(w * 256.0, (double)s.p);
+            }
+            const double bound = (2.0 * L - 1) * L * (double)d.k * 16384.0 * (double)(s.p - 1);
+            d.lazy = bound < 9007199254740992.0 ? 1 : 0;
+        }
         d.tiles_n = (d.n + d.tile - 1) / d.tile;
     }
     // most expensive tiles first, within each class
     auto cost = [&](int64_t i) { return (int64_t)hj[i].tile * hj[i].tile * hj[i].k; };
     std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
     auto cls_of = [&](const BJob &d) {
+        if (d.L > 0) return 3 + d.L;
         const bool exact = (int64_t)d.R >= ((int64_t)d.k + 3) / 4;
         return (d.tile == 64 ? 0 : 2) + (exact ? 0 : 1);
     };

@@ -1,0 +1,70 @@
+"""Batched products in both arithmetic forms: the int8 digit form (default for
+k <= 16384, one class per digit count 1..4) and the FP64 form (forced with
+MODP_BATCHED_FP64=1, and used for deeper contractions), each product equal to
+the oracle's."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make_jobs(g, specs, fill=None, seed=0):
+    jobs, host = [], []
+    for i, (m, n, k, p) in enumerate(specs):
+        if fill is None:
+            Ah = O.draw_matrix(m, k, p, seed=seed * 1000 + i)
+            Bh = O.draw_matrix(k, n, p, seed=seed * 1000 + i, offset=m * k)
+        else:
+            Ah = np.full((m, k), fill(p), dtype=np.int64)
+            Bh = np.full((k, n), fill(p), dtype=np.int64)
+        jobs.append((g.CuModMatrix(Ah, p, elem_type=np.float64), g.CuModMatrix(Bh, p, elem_type=np.float64),
+                     g.zeros(np.float64, m, n, p), p))
+        host.append((Ah, Bh, p))
+    return jobs, host
+
+
+def check(g, jobs, host):
+    g.gemm_batched(jobs)
+    g.context().synchronize()
+    for i, ((_, _, C, _), (Ah, Bh, p)) in enumerate(zip(jobs, host)):
+        assert np.array_equal(g.Array(C).astype(np.int64), O.gemm(Ah, Bh, p)), f"product {i} (p={p})"
+
+
+P4 = O.largest_primes_below(2 ** 25, 1)[0]
+PRIMES = [2, 3, 251, 257, 65521, 65537, 8388593, P4, 2 ** 26 - 5]
+
+
+@pytest.mark.parametrize("p", PRIMES)
+def test_digit_form_each_digit_count(gffm, p):
+    rng = np.random.default_rng(p % 1000)
+    specs = [(int(rng.integers(1, 200)), int(rng.integers(1, 200)), int(rng.integers(1, 300)), p) for _ in range(12)]
+    specs += [(64, 64, 64, p), (65, 63, 129, p), (1, 1, 1, p), (128, 7, 2000, p)]
+    print(f"digit form p={p}", flush=True)
+    check(gffm, *make_jobs(gffm, specs, seed=5))
+
+
+@pytest.mark.parametrize("fill", ["pm1", "half"])
+@pytest.mark.parametrize("p", [65521, 8388593, 2 ** 26 - 5])
+def test_digit_form_extreme_residues(gffm, p, fill):
+    f = (lambda q: q - 1) if fill == "pm1" else (lambda q: (q - 1) // 2)
+    specs = [(70, 66, 3000, p), (33, 40, 256, p), (64, 64, 16384, p)]
+    print(f"digit form extremes p={p} fill={fill}", flush=True)
+    check(gffm, *make_jobs(gffm, specs, fill=f))
+
+
+def test_fp64_form_forced_vs_oracle(gffm, monkeypatch):
+    monkeypatch.setenv("MODP_BATCHED_FP64", "1")
+    rng = np.random.default_rng(3)
+    primes = O.largest_primes_below(2 ** 23, 64)
+    specs = [(int(rng.integers(16, 257)), int(rng.integers(16, 257)), int(rng.integers(16, 257)),
+              primes[int(rng.integers(0, 64))]) for _ in range(120)]
+    specs += [(50, 40, 600, 2 ** 26 - 5), (3, 5, 7, 65521)]
+    check(gffm, *make_jobs(gffm, specs, seed=7))
+
+
+def test_deep_contraction_takes_fp64_form(gffm):
+    # k above the digit form's 16384 goes to the FP64 classes, in the same plan
+    specs = [(20, 18, 17000, 65521), (9, 30, 16385, 2 ** 26 - 5), (40, 40, 100, 8388593)]
+    check(gffm, *make_jobs(gffm, specs, seed=11))
