@@ -106,9 +106,9 @@ __global__ __launch_bounds__(256, 2) void k_batched_gemm(const BJob *__restrict_
 }
 
 // Digit form of one work item: a 64 x 64 output tile on four waves (32 x 32
-// blocks, v_mfma_i32_32x32x32_i8).  Per 64-deep stage every thread loads 16
-// elements of A (one row, the lanes along the rows: 512-byte runs) and 16 of
-// B (lane pairs of consecutive k, the lanes along k: two 512-byte column runs
+// blocks, v_mfma_i32_32x32x32_i8).  Per 32-deep stage every thread loads 8
+// elements of A (one row, the lanes along the rows: 512-byte runs) and 8 of B
+// (lane pairs of consecutive k, the lanes along k: four 256-byte column runs
 // per load), turns each residue into L signed base-256 digits (the limb
 // form's representation) and writes them to the digit planes in LDS; the
 // next stage's loads are in flight during the MFMAs.  The L x L digit
@@ -116,15 +116,16 @@ __global__ __launch_bounds__(256, 2) void k_batched_gemm(const BJob *__restrict_
 // (|sum| <= L k 2^14 < 2^31 for k <= 16384), and the epilogue forms
 // sum_s (acc_s mod p) (256^s mod p) mod p in FP64.  The blocks are computed
 // transposed (B digits as the first operand), so lane l holds row l % 32 of
-// C and a store writes 32 consecutive rows of a column.  LDS rows hold 64
-// digits (4 chunks); chunk c of row r sits at slot c ^ ((r >> 2) & 3), which
+// C and a store writes 32 consecutive rows of a column.  LDS rows hold 32
+// digits (2 chunks); chunk c of row r sits at slot c ^ ((r >> 3) & 1), which
 // keeps the 16-lane fragment reads free of bank conflicts.
-constexpr int BL_TILE = 64, BL_BK = 64, BL_KMAX = 16384;
+constexpr int BL_TILE = 64, BL_BK = 32, BL_KMAX = 16384;
 template <typename T, int L>
 __global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict__ jobs,
                                                          const BItem *__restrict__ items, XcdRanges xr,
                                                          int *__restrict__ counter)
 {
+    typedef int v2i __attribute__((ext_vector_type(2)));
     typedef int v4i __attribute__((ext_vector_type(4)));
     typedef int v16i __attribute__((ext_vector_type(16)));
     constexpr int TL = BL_TILE, BK = BL_BK, NA = 2 * L - 1;
@@ -134,12 +135,12 @@ __global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict_
     __shared__ int s_item;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave & 1, wc = wave >> 1;
-    // A staging: row sr, k chunk sc (16 k); B staging: column 16 wave + 2 i +
-    // lane / 32 (i < 8), k pair 2 (lane % 32)
+    // A staging: row sr, k quarter sc (8 k); B staging: column 16 wave +
+    // 4 i + lane / 16 (i < 4), k pair 2 (lane % 16)
     const int sr = tid & 63, sc = tid >> 6;
-    const int aslot = sr * BK + 16 * (sc ^ ((sr >> 2) & 3));
-    const int bk2 = 2 * (lane & 31);
-    const int fr = lane & 31, h = lane >> 5, fsw = (fr >> 2) & 3;
+    const int aslot = sr * BK + 16 * ((sc >> 1) ^ ((sr >> 3) & 1)) + 8 * (sc & 1);
+    const int bk2 = 2 * (lane & 15), bq = lane >> 4;
+    const int fr = lane & 31, h = lane >> 5, fsw = (fr >> 3) & 1;
     const int home = blockIdx.x % NXCD;
     for (int qq = 0; qq < NXCD; ++qq) {
         const int q = (home + qq) % NXCD;
@@ -160,17 +161,17 @@ __global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict_
             const int K = j.k, ns = (K + BK - 1) / BK;
             const int64_t arow = r0 + sr;
             const bool arow_ok = arow < j.m;
-            T va[16], vb[16];
+            T va[8], vb[8];
             auto gload = [&](int st) {
                 const int64_t kb = (int64_t)st * BK;
 #pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const int64_t kk = kb + 16 * sc + e;
+                for (int e = 0; e < 8; ++e) {
+                    const int64_t kk = kb + 8 * sc + e;
                     va[e] = (arow_ok && kk < K) ? A[arow + kk * j.lda] : (T)0;
                 }
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int64_t col = c0 + 16 * wave + 2 * i + h;
+                for (int i = 0; i < 4; ++i) {
+                    const int64_t col = c0 + 16 * wave + 4 * i + bq;
 #pragma unroll
                     for (int t = 0; t < 2; ++t) {
                         const int64_t kk = kb + bk2 + t;
@@ -179,22 +180,22 @@ __global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict_
                 }
             };
             auto lstore = [&](int buf) {
-                v4i wa[L];
+                v2i wa[L];
 #pragma unroll
-                for (int u = 0; u < L; ++u) wa[u] = v4i{0, 0, 0, 0};
+                for (int u = 0; u < L; ++u) wa[u] = v2i{0, 0};
 #pragma unroll
-                for (int e = 0; e < 16; ++e) {
+                for (int e = 0; e < 8; ++e) {
                     int32_t xa = digit_rep(to_res<T>(va[e], mp), TOP, mp);
 #pragma unroll
                     for (int u = 0; u < L; ++u)
                         wa[u][e >> 2] |= (int)(((uint32_t)next_digit(xa) & 255u) << (8 * (e & 3)));
                 }
 #pragma unroll
-                for (int u = 0; u < L; ++u) *reinterpret_cast<v4i *>(&As[buf][u][aslot]) = wa[u];
+                for (int u = 0; u < L; ++u) *reinterpret_cast<v2i *>(&As[buf][u][aslot]) = wa[u];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int c = 16 * wave + 2 * i + h;
-                    const int off = c * BK + 16 * ((bk2 >> 4) ^ ((c >> 2) & 3)) + (bk2 & 15);
+                for (int i = 0; i < 4; ++i) {
+                    const int c = 16 * wave + 4 * i + bq;
+                    const int off = c * BK + 16 * ((bk2 >> 4) ^ ((c >> 3) & 1)) + (bk2 & 15);
                     int32_t x0 = digit_rep(to_res<T>(vb[2 * i], mp), TOP, mp);
                     int32_t x1 = digit_rep(to_res<T>(vb[2 * i + 1], mp), TOP, mp);
 #pragma unroll
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict_
                 if (st + 1 < ns) gload(st + 1);
 #pragma unroll
                 for (int kk = 0; kk < BK / 32; ++kk) {
-                    const int off = 16 * ((2 * kk + h) ^ fsw);
+                    const int off = 16 * ((2 * kk + h) ^ fsw);  // BK = 32: kk = 0, chunk h
                     v4i fa[L], fb[L];
 #pragma unroll
                     for (int u = 0; u < L; ++u) {
@@ -258,9 +259,10 @@ void launch_class(int cls, const modp_gemm_plan *pl, hipStream_t st)
     const BItem *it = pl->ditems + pl->cls_off[cls];
     int *ctr = pl->dcounter + cls * 16 * NXCD;
     const XcdRanges &xr = pl->xr[cls];
-    // resident workgroups per CU by the kernels' register and LDS use: 3
-    // for 64x64 FP64 tiles, 5 for 32x32, 2 for the digit form
-    const int64_t cap = (int64_t)pl->grid * (cls >= 4 ? 2 : (cls < 2 ? 3 : 5));
+    // resident workgroups per CU by the kernels' register use: 3 for 64x64
+    // FP64 tiles, 5 for 32x32; 4, 3, 2, 2 for the digit form with 1-4 digits
+    static const int per_cu[NCLASS] = {3, 3, 5, 5, 4, 3, 2, 2};
+    const int64_t cap = (int64_t)pl->grid * per_cu[cls];
     // a multiple of the XCD count, so every XCD gets the same number
     int64_t g64 = n < cap ? n : cap;
     g64 = (g64 + NXCD - 1) / NXCD * NXCD;
