@@ -179,29 +179,45 @@ __global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict_
                     }
                 }
             };
+            // residue of a stored element: a compare when it is already in
+            // [0, p) (the usual case), the FP64 reduction otherwise
+            auto res = [&](T v) -> int32_t {
+                const double x = (double)v;
+                uint32_t r = (uint32_t)x;
+                if (!(x >= 0.0 && x < mp.pd)) r = (uint32_t)dmod(x, mp);
+                return digit_rep(r, TOP, mp);
+            };
+            // Digits: the stored byte of digit u of x is the low byte of x_u,
+            // with x_0 = x and x_{u+1} = (x_u + 128) >> 8 (arithmetic), the
+            // signed base-256 digits of the limb form.
             auto lstore = [&](int buf) {
-                v2i wa[L];
+                int32_t xa[8];
 #pragma unroll
-                for (int u = 0; u < L; ++u) wa[u] = v2i{0, 0};
+                for (int e = 0; e < 8; ++e) xa[e] = res(va[e]);
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    int32_t xa = digit_rep(to_res<T>(va[e], mp), TOP, mp);
+                for (int u = 0; u < L; ++u) {
+                    v2i wa;
 #pragma unroll
-                    for (int u = 0; u < L; ++u)
-                        wa[u][e >> 2] |= (int)(((uint32_t)next_digit(xa) & 255u) << (8 * (e & 3)));
+                    for (int d = 0; d < 2; ++d)
+                        wa[d] = (int)(((uint32_t)xa[4 * d] & 255u) | (((uint32_t)xa[4 * d + 1] & 255u) << 8) |
+                                      (((uint32_t)xa[4 * d + 2] & 255u) << 16) | ((uint32_t)xa[4 * d + 3] << 24));
+                    *reinterpret_cast<v2i *>(&As[buf][u][aslot]) = wa;
+                    if (u + 1 < L) {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) xa[e] = (xa[e] + 128) >> 8;
+                    }
                 }
-#pragma unroll
-                for (int u = 0; u < L; ++u) *reinterpret_cast<v2i *>(&As[buf][u][aslot]) = wa[u];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int c = 16 * wave + 4 * i + bq;
                     const int off = c * BK + 16 * ((bk2 >> 4) ^ ((c >> 3) & 1)) + (bk2 & 15);
-                    int32_t x0 = digit_rep(to_res<T>(vb[2 * i], mp), TOP, mp);
-                    int32_t x1 = digit_rep(to_res<T>(vb[2 * i + 1], mp), TOP, mp);
+                    int32_t x0 = res(vb[2 * i]), x1 = res(vb[2 * i + 1]);
 #pragma unroll
                     for (int u = 0; u < L; ++u) {
-                        const uint32_t d0 = (uint32_t)next_digit(x0) & 255u, d1 = (uint32_t)next_digit(x1) & 255u;
-                        *reinterpret_cast<uint16_t *>(&Bs[buf][u][off]) = (uint16_t)(d0 | (d1 << 8));
+                        *reinterpret_cast<uint16_t *>(&Bs[buf][u][off]) =
+                            (uint16_t)(((uint32_t)x0 & 255u) | (((uint32_t)x1 & 255u) << 8));
+                        x0 = (x0 + 128) >> 8;
+                        x1 = (x1 + 128) >> 8;
                     }
                 }
             };
