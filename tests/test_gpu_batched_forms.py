@@ -68,3 +68,24 @@ def test_deep_contraction_takes_fp64_form(gffm):
     # k above the digit form's 16384 goes to the FP64 classes, in the same plan
     specs = [(20, 18, 17000, 65521), (9, 30, 16385, 2 ** 26 - 5), (40, 40, 100, 8388593)]
     check(gffm, *make_jobs(gffm, specs, seed=11))
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_unreduced_inputs_are_reduced(gffm, monkeypatch, fp64):
+    # stored values outside [0, p) (negative, >= p, up to 2^40) are reduced
+    # first, as in the single-product path
+    if fp64:
+        monkeypatch.setenv("MODP_BATCHED_FP64", "1")
+    g = gffm
+    rng = np.random.default_rng(17)
+    jobs, host = [], []
+    for i, p in enumerate([101, 65521, 8388593, 2 ** 26 - 5]):
+        m, n, k = 37 + i, 29 + 3 * i, 70 + 11 * i
+        Ah = rng.integers(-(2 ** 40), 2 ** 40, size=(m, k))
+        Bh = rng.integers(-3 * p, 3 * p, size=(k, n))
+        Ah[:, ::3] = np.mod(Ah[:, ::3], p)  # some columns in range
+        A = g.CuModMatrix(Ah, p, mod=False, elem_type=np.float64)
+        B = g.CuModMatrix(Bh, p, mod=False, elem_type=np.float64)
+        jobs.append((A, B, g.zeros(np.float64, m, n, p), p))
+        host.append((np.mod(Ah, p), np.mod(Bh, p), p))
+    check(g, jobs, host)
