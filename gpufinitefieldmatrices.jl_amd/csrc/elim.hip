@@ -249,10 +249,11 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             if (SMALL) v[q][jc] = dmod_near(v[q][jc], mp);
-            if ((elig >> q) & 1u) {
-                const key_t k = K::make((uint32_t)v[q][jc], pos[q]);
-                if (k > mine) { mine = k; mq = q; }
-            }
+            // selects, not branches: keys of rows that are not eligible are 0
+            const key_t k = ((elig >> q) & 1u) ? K::make((uint32_t)v[q][jc], pos[q]) : (key_t)0;
+            const bool better = k > mine;
+            mine = better ? k : mine;
+            mq = better ? q : mq;
         }
         // every lane fetches the inverse of its own candidate before the wave
         // reduction, so the table load overlaps it
@@ -270,9 +271,11 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
                 s_cinv[par][wave] = invtab ? myinv : modinv_fast(K::val(wbest), mp);
                 s_cidx[par][wave] = rowi(mq);
             }
+            // one divergent branch (the winning lane), then scalar branches
+            // on the wave-uniform register row
             auto publish = [&](auto Q) __attribute__((always_inline)) {
                 constexpr int q = decltype(Q)::value;
-                if (wq == q && lane == wl) {
+                if (wq == q) {
 #pragma unroll
                     for (int j = 0; j < NBW; ++j)
                         if (j > jc) s_crow[par][wave][j] = v[q][j];
@@ -281,7 +284,7 @@ __global__ __launch_bounds__(PT) void k_panel(int32_t *__restrict__ W, int64_t l
                     asm volatile("" ::: "memory");
                 }
             };
-            Unroll<0, RPT>::run(publish);
+            if (lane == wl) Unroll<0, RPT>::run(publish);
         }
         if (lane == 0) s_key[par][wave] = wbest;
         __syncthreads();
