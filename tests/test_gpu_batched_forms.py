@@ -89,3 +89,23 @@ def test_unreduced_inputs_are_reduced(gffm, monkeypatch, fp64):
         jobs.append((A, B, g.zeros(np.float64, m, n, p), p))
         host.append((np.mod(Ah, p), np.mod(Bh, p), p))
     check(g, jobs, host)
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_float32_storage(gffm, monkeypatch, fp64):
+    # residues stored as float32; the reference's Float32 rule admits only
+    # moduli whose products fit the 24-bit significand (stripe_mul.jl:200-202)
+    if fp64:
+        monkeypatch.setenv("MODP_BATCHED_FP64", "1")
+    g = gffm
+    jobs, host = [], []
+    for i, p in enumerate([2, 7, 251, O.largest_primes_below(2898, 1)[0]]):
+        m, n, k = 40 + 7 * i, 33 + 5 * i, 90 + 13 * i
+        Ah = O.draw_matrix(m, k, p, seed=60 + i)
+        Bh = O.draw_matrix(k, n, p, seed=60 + i, offset=m * k)
+        jobs.append((g.CuModMatrix(Ah, p, elem_type=np.float32), g.CuModMatrix(Bh, p, elem_type=np.float32),
+                     g.zeros(np.float32, m, n, p), p))
+        host.append((Ah, Bh, p))
+    g.gemm_batched(jobs, elem_type=np.float32)
+    for i, ((_, _, C, _), (Ah, Bh, p)) in enumerate(zip(jobs, host)):
+        assert np.array_equal(g.Array(C).astype(np.int64), O.gemm(Ah, Bh, p)), f"product {i} (p={p})"
