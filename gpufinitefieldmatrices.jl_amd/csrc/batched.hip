@@ -255,13 +255,18 @@ __global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict_
                 const int64_t jj = c0 + wc * 32 + 8 * (e >> 2) + 4 * h + (e & 3);
                 double S = 0.0;
                 if (j.lazy) {
+                    // |S| < 2^51: the rounded quotient is within one of the
+                    // exact one, so one fix-up gives S mod p
 #pragma unroll
                     for (int t = 0; t < NA; ++t) S = fma((double)acc[t][e], j.w[t], S);
+                    S = fma(-rint(S * mp.pinv), mp.pd, S);
+                    S = S < 0.0 ? S + mp.pd : S;
                 } else {
 #pragma unroll
                     for (int t = 0; t < NA; ++t) S = dmod(fma(dmod((double)acc[t][e], mp), j.w[t], S), mp);
                 }
-                if (i < j.m && jj < j.n) C[i + jj * j.ldc] = from_res<T>((uint32_t)dmod(S, mp));
+                // S is an exact integer in [0, p)
+                if (i < j.m && jj < j.n) C[i + jj * j.ldc] = (T)S;
             }
         }
     }
@@ -427,8 +432,8 @@ int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count
         d.lazy = 0;
         const int L = limb_count(s.p);
         if (!fp64_only && d.k <= BL_KMAX && L <= 4) {
-            // digit form: exact when every term of the recombination,
-            // |acc_s| (256^s mod p) with |acc_s| <= L k 2^14, sums below 2^53
+            // digit form: the recombination reduces once at the end when its
+            // terms |acc_s| (256^s mod p), |acc_s| <= L k 2^14, sum below 2^51
             d.L = L;
             d.tile = BL_TILE;
             double w = 1.0;
@@ -437,7 +442,7 @@ int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count
                 w = std::fmod(w * 256.0, (double)s.p);
             }
             const double bound = (2.0 * L - 1) * L * (double)d.k * 16384.0 * (double)(s.p - 1);
-            d.lazy = bound < 9007199254740992.0 ? 1 : 0;
+            d.lazy = bound < 2251799813685248.0 ? 1 : 0;  // 2^51
         }
         d.tiles_n = (d.n + d.tile - 1) / d.tile;
     }
