@@ -52,7 +52,9 @@ struct BItem {
 // XCD w mod 8) take their tiles first, so the tiles of one product share
 // its operands in one L2; a workgroup whose queue is empty then helps the
 // others.
-constexpr int NCLASS = 8;  // 0-3 FP64 form, 4-7 digit form with 1-4 digits
+// 0-3 FP64 form, 4-7 digit form with 1-4 digits (a 64 x 64 tile per item),
+// 8-11 digit form with 1-4 digits for k <= 256 (a 64-row panel per item)
+constexpr int NCLASS = 12;
 constexpr int NXCD = 8;
 struct XcdRanges {
     int64_t off[NXCD + 1];  // item ranges of the XCD queues within a class
@@ -272,6 +274,162 @@ __global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict_
     }
 }
 
+// Digit form, one 64-row panel per work item (k <= 256): the panel's rows of
+// A are split into digits during its first column tile, stage by stage in the
+// same software pipeline as B, and kept in LDS for the later column tiles
+// (rows of 256 + 16 bytes: 16 consecutive rows read at one offset fall in
+// distinct bank groups).  The A digits of a product are computed once per row
+// panel instead of once per tile.
+constexpr int RP_KMAX = 256;
+template <typename T, int L>
+__global__ __launch_bounds__(256, L == 4 ? 1 : 2) void k_batched_limb_rp(const BJob *__restrict__ jobs,
+                                                            const BItem *__restrict__ items, XcdRanges xr,
+                                                            int *__restrict__ counter)
+{
+    typedef int v2i __attribute__((ext_vector_type(2)));
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    typedef int v16i __attribute__((ext_vector_type(16)));
+    constexpr int TL = BL_TILE, BK = 32, NA = 2 * L - 1, AP = RP_KMAX + 16;
+    constexpr int32_t TOP = (int32_t)(127 * ((((int64_t)1) << (8 * L)) - 1) / 255);
+    __shared__ __attribute__((aligned(16))) int8_t As[L][TL * AP];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[2][L][TL * BK];
+    __shared__ int s_item;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wc = wave >> 1;
+    const int sr = tid & 63, sc = tid >> 6;
+    const int bk2 = 2 * (lane & 15), bq = lane >> 4;
+    const int fr = lane & 31, h = lane >> 5, fsw = (fr >> 3) & 1;
+    const int home = blockIdx.x % NXCD;
+    for (int qq = 0; qq < NXCD; ++qq) {
+        const int q = (home + qq) % NXCD;
+        const int64_t base = xr.off[q], cnt = xr.off[q + 1] - base;
+        if (cnt <= 0) continue;
+        for (;;) {
+            if (threadIdx.x == 0) s_item = atomicAdd(counter + 16 * q, 1);
+            __syncthreads();
+            const int it = s_item;
+            __syncthreads();
+            if (it >= cnt) break;
+            const BItem item = items[base + it];
+            const BJob &j = jobs[item.job];
+            const int64_t r0 = (int64_t)item.tile * TL;
+            const T *A = (const T *)j.A, *B = (const T *)j.B;
+            const ModP mp = j.mp;
+            const int K = j.k, ns = (K + BK - 1) / BK;
+            auto res = [&](T v) -> int32_t {
+                const double x = (double)v;
+                uint32_t r = (uint32_t)x;
+                if (!(x >= 0.0 && x < mp.pd)) r = (uint32_t)dmod(x, mp);
+                return digit_rep(r, TOP, mp);
+            };
+            T *C = (T *)j.C;
+            const int64_t i = r0 + wr * 32 + fr;
+            const int64_t arow = r0 + sr;
+            const bool arow_ok = arow < j.m;
+            for (int tj = 0; tj < j.tiles_n; ++tj) {
+                const int64_t c0 = (int64_t)tj * TL;
+                // the first column tile also stages the A panel: thread (row
+                // sr, quarter sc) splits k = 32 st + 8 sc .. + 7 of each stage
+                const bool first = tj == 0;
+                T va[8], vb[8];
+                auto gload = [&](int st) {
+                    const int64_t kb = (int64_t)st * BK;
+                    if (first) {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            const int64_t kk = kb + 8 * sc + e;
+                            va[e] = (arow_ok && kk < K) ? A[arow + kk * j.lda] : (T)0;
+                        }
+                    }
+#pragma unroll
+                    for (int ii = 0; ii < 4; ++ii) {
+                        const int64_t col = c0 + 16 * wave + 4 * ii + bq;
+#pragma unroll
+                        for (int t = 0; t < 2; ++t) {
+                            const int64_t kk = kb + bk2 + t;
+                            vb[2 * ii + t] = (col < j.n && kk < K) ? B[kk + col * j.ldb] : (T)0;
+                        }
+                    }
+                };
+                auto lstore = [&](int buf, int st) {
+                    if (first) {
+                        int32_t xa[8];
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) xa[e] = res(va[e]);
+#pragma unroll
+                        for (int u = 0; u < L; ++u) {
+                            v2i wa;
+#pragma unroll
+                            for (int d = 0; d < 2; ++d)
+                                wa[d] = (int)(((uint32_t)xa[4 * d] & 255u) | (((uint32_t)xa[4 * d + 1] & 255u) << 8) |
+                                              (((uint32_t)xa[4 * d + 2] & 255u) << 16) |
+                                              ((uint32_t)xa[4 * d + 3] << 24));
+                            *reinterpret_cast<v2i *>(&As[u][sr * AP + st * BK + 8 * sc]) = wa;
+                            if (u + 1 < L) {
+#pragma unroll
+                                for (int e = 0; e < 8; ++e) xa[e] = (xa[e] + 128) >> 8;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int ii = 0; ii < 4; ++ii) {
+                        const int c = 16 * wave + 4 * ii + bq;
+                        const int off = c * BK + 16 * ((bk2 >> 4) ^ ((c >> 3) & 1)) + (bk2 & 15);
+                        int32_t x0 = res(vb[2 * ii]), x1 = res(vb[2 * ii + 1]);
+#pragma unroll
+                        for (int u = 0; u < L; ++u) {
+                            *reinterpret_cast<uint16_t *>(&Bs[buf][u][off]) =
+                                (uint16_t)(((uint32_t)x0 & 255u) | (((uint32_t)x1 & 255u) << 8));
+                            x0 = (x0 + 128) >> 8;
+                            x1 = (x1 + 128) >> 8;
+                        }
+                    }
+                };
+                v16i acc[NA];
+#pragma unroll
+                for (int t = 0; t < NA; ++t) acc[t] = v16i{0};
+                if (ns > 0) {
+                    gload(0);
+                    lstore(0, 0);
+                }
+                __syncthreads();
+                for (int st = 0; st < ns; ++st) {
+                    const int buf = st & 1;
+                    if (st + 1 < ns) gload(st + 1);
+                    v4i fa[L], fb[L];
+#pragma unroll
+                    for (int u = 0; u < L; ++u) {
+                        fa[u] = *reinterpret_cast<const v4i *>(&As[u][(wr * 32 + fr) * AP + st * BK + 16 * h]);
+                        fb[u] = *reinterpret_cast<const v4i *>(&Bs[buf][u][(wc * 32 + fr) * BK + 16 * (h ^ fsw)]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < L; ++u)
+#pragma unroll
+                        for (int v = 0; v < L; ++v)
+                            acc[u + v] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[v], fa[u], acc[u + v], 0, 0, 0);
+                    if (st + 1 < ns) lstore(buf ^ 1, st + 1);
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int64_t jj = c0 + wc * 32 + 8 * (e >> 2) + 4 * h + (e & 3);
+                    double S = 0.0;
+                    if (j.lazy) {
+#pragma unroll
+                        for (int t = 0; t < NA; ++t) S = fma((double)acc[t][e], j.w[t], S);
+                        S = fma(-rint(S * mp.pinv), mp.pd, S);
+                        S = S < 0.0 ? S + mp.pd : S;
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < NA; ++t) S = dmod(fma(dmod((double)acc[t][e], mp), j.w[t], S), mp);
+                    }
+                    if (i < j.m && jj < j.n) C[i + jj * j.ldc] = (T)S;
+                }
+            }
+        }
+    }
+}
+
 template <typename T>
 void launch_class(int cls, const modp_gemm_plan *pl, hipStream_t st)
 {
@@ -280,9 +438,10 @@ void launch_class(int cls, const modp_gemm_plan *pl, hipStream_t st)
     const BItem *it = pl->ditems + pl->cls_off[cls];
     int *ctr = pl->dcounter + cls * 16 * NXCD;
     const XcdRanges &xr = pl->xr[cls];
-    // resident workgroups per CU by the kernels' register use: 3 for 64x64
-    // FP64 tiles, 5 for 32x32; 4, 3, 2, 2 for the digit form with 1-4 digits
-    static const int per_cu[NCLASS] = {3, 3, 5, 5, 4, 3, 2, 2};
+    // resident workgroups per CU by the kernels' register and LDS use: 3 for
+    // 64x64 FP64 tiles, 5 for 32x32; 4, 3, 2, 2 for the digit form with 1-4
+    // digits; 4, 3, 2, 1 for its row-panel kernel
+    static const int per_cu[NCLASS] = {3, 3, 5, 5, 4, 3, 2, 2, 4, 3, 2, 1};
     const int64_t cap = (int64_t)pl->grid * per_cu[cls];
     // a multiple of the XCD count, so every XCD gets the same number
     int64_t g64 = n < cap ? n : cap;
@@ -296,7 +455,11 @@ void launch_class(int cls, const modp_gemm_plan *pl, hipStream_t st)
     case 4: k_batched_limb<T, 1><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
     case 5: k_batched_limb<T, 2><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
     case 6: k_batched_limb<T, 3><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
-    default: k_batched_limb<T, 4><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    case 7: k_batched_limb<T, 4><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    case 8: k_batched_limb_rp<T, 1><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    case 9: k_batched_limb_rp<T, 2><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    case 10: k_batched_limb_rp<T, 3><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
+    default: k_batched_limb_rp<T, 4><<<g, 256, 0, st>>>(pl->djobs, it, xr, ctr); break;
     }
 }
 
@@ -414,6 +577,10 @@ int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count
     // compare the two forms)
     const char *fenv = getenv("MODP_BATCHED_FP64");
     const bool fp64_only = fenv && atoi(fenv) != 0;
+    // MODP_BATCHED_TILES=1 keeps digit-form products on 64 x 64 tile items
+    // (no row panels)
+    const char *tenv = getenv("MODP_BATCHED_TILES");
+    const bool tile_items = tenv && atoi(tenv) != 0;
     modp_gemm_plan *pl = new modp_gemm_plan();
     pl->kind = kind;
     pl->count = count;
@@ -450,7 +617,7 @@ int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count
     auto cost = [&](int64_t i) { return (int64_t)hj[i].tile * hj[i].tile * hj[i].k; };
     std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
     auto cls_of = [&](const BJob &d) {
-        if (d.L > 0) return 3 + d.L;
+        if (d.L > 0) return (d.k <= RP_KMAX && !tile_items ? 7 : 3) + d.L;
         const bool exact = (int64_t)d.R >= ((int64_t)d.k + 3) / 4;
         return (d.tile == 64 ? 0 : 2) + (exact ? 0 : 1);
     };
@@ -463,12 +630,14 @@ int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count
             const BJob &d = hj[oi];
             if (d.m > 0 && d.n > 0 && cls_of(d) == c) prods.push_back(oi);
         }
-        auto tiles_of = [&](const BJob &d) { return ((d.m + d.tile - 1) / d.tile) * d.tiles_n; };
+        // items: output tiles, or row panels for the row-panel classes
+        auto rows_of = [&](const BJob &d) { return (d.m + d.tile - 1) / d.tile; };
+        auto tiles_of = [&](const BJob &d) { return c >= 8 ? rows_of(d) : rows_of(d) * d.tiles_n; };
         std::vector<int64_t> load(NXCD, 0);
         std::vector<std::vector<int64_t>> per(NXCD);
         for (int64_t oi : prods) {
             const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            load[x] += cost(oi) * tiles_of(hj[oi]);
+            load[x] += cost(oi) * rows_of(hj[oi]) * hj[oi].tiles_n;
             per[x].push_back(oi);
         }
         for (int x = 0; x < NXCD; ++x) {
@@ -500,7 +669,7 @@ int plan_build(modp_ctx *ctx, int kind, const modp_gemm_job *jobs, int64_t count
     // capture [counter reset, one persistent kernel per class] in a graph
     e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        k_queue_reset<<<1, 64, 0, ctx->stream>>>(pl->dcounter);
+        k_queue_reset<<<1, 128, 0, ctx->stream>>>(pl->dcounter);
         for (int c = 0; c < NCLASS; ++c) {
             if (kind == EK_F64) launch_class<double>(c, pl, ctx->stream);
             else launch_class<float>(c, pl, ctx->stream);

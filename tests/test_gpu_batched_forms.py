@@ -1,5 +1,6 @@
 """Batched products in both arithmetic forms: the int8 digit form (default for
-k <= 16384, one class per digit count 1..4) and the FP64 form (forced with
+k <= 16384, one class per digit count 1..4, with one 64-row panel per work
+item for k <= 256 and 64 x 64 tiles otherwise) and the FP64 form (forced with
 MODP_BATCHED_FP64=1, and used for deeper contractions), each product equal to
 the oracle's."""
 import numpy as np
@@ -52,6 +53,21 @@ def test_digit_form_extreme_residues(gffm, p, fill):
     specs = [(70, 66, 3000, p), (33, 40, 256, p), (64, 64, 16384, p)]
     print(f"digit form extremes p={p} fill={fill}", flush=True)
     check(gffm, *make_jobs(gffm, specs, fill=f))
+
+
+@pytest.mark.parametrize("tiles", ["0", "1"])
+def test_digit_form_row_panels(gffm, monkeypatch, tiles):
+    # k <= 256: one item per 64-row panel, whose A digits stay in LDS over all
+    # column tiles; MODP_BATCHED_TILES=1 puts the same products on 64 x 64 tiles
+    monkeypatch.setenv("MODP_BATCHED_TILES", tiles)
+    specs = []
+    for p in [3, 251, 65521, 8388593, 2 ** 26 - 5]:
+        specs += [(300, 500, 256, p), (129, 65, 1, p), (64, 1000, 33, p), (200, 200, 257, p), (7, 130, 255, p)]
+    print(f"row panels tiles={tiles}", flush=True)
+    check(gffm, *make_jobs(gffm, specs, seed=13))
+    fill = [(lambda q: q - 1), (lambda q: (q - 1) // 2)]
+    for f in fill:
+        check(gffm, *make_jobs(gffm, [(130, 300, 256, 2 ** 26 - 5), (65, 66, 256, 65521)], fill=f))
 
 
 def test_fp64_form_forced_vs_oracle(gffm, monkeypatch):
