@@ -122,14 +122,98 @@ __global__ __launch_bounds__(256, 2) void k_batched_gemm(const BJob *__restrict_
 // digits (2 chunks); chunk c of row r sits at slot c ^ ((r >> 3) & 1), which
 // keeps the 16-lane fragment reads free of bank conflicts.
 constexpr int BL_TILE = 64, BL_BK = 32, BL_KMAX = 16384;
+typedef int bl_v2i __attribute__((ext_vector_type(2)));
+typedef int bl_v16i __attribute__((ext_vector_type(16)));
+
+// Residue of a stored element in the digit representation (top as in
+// digit_rep): a compare when it is already in [0, p) (the usual case), the
+// FP64 reduction otherwise.
+template <typename T>
+__device__ __forceinline__ int32_t bl_residue(T v, const ModP &mp, int32_t top)
+{
+    const double x = (double)v;
+    uint32_t r = (uint32_t)x;
+    if (!(x >= 0.0 && x < mp.pd)) r = (uint32_t)dmod(x, mp);
+    return digit_rep(r, top, mp);
+}
+
+// Digits: the stored byte of digit u of x is the low byte of x_u, with
+// x_0 = x and x_{u+1} = (x_u + 128) >> 8 (arithmetic), the signed base-256
+// digits of the limb form.  Eight consecutive k of one A row: digit u goes to
+// dst(u) as one 8-byte word.
+template <int L, class Dst> __device__ __forceinline__ void bl_store_a8(int32_t (&x)[8], Dst dst)
+{
+#pragma unroll
+    for (int u = 0; u < L; ++u) {
+        bl_v2i w;
+#pragma unroll
+        for (int d = 0; d < 2; ++d)
+            w[d] = (int)(((uint32_t)x[4 * d] & 255u) | (((uint32_t)x[4 * d + 1] & 255u) << 8) |
+                         (((uint32_t)x[4 * d + 2] & 255u) << 16) | ((uint32_t)x[4 * d + 3] << 24));
+        *reinterpret_cast<bl_v2i *>(dst(u)) = w;
+        if (u + 1 < L) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[e] = (x[e] + 128) >> 8;
+        }
+    }
+}
+
+// B stage of 32 k into the digit planes Bs[u] (64 columns of 32 bytes, chunk
+// c of column r at slot c ^ ((r >> 3) & 1)): element 2 i + t of vb is k pair
+// bk2 + t of column 16 wave + 4 i + bq.
+template <typename T, int L>
+__device__ __forceinline__ void bl_store_b(const T (&vb)[8], int8_t (*Bs)[BL_TILE * BL_BK], int wave, int bq,
+                                           int bk2, const ModP &mp, int32_t top)
+{
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int c = 16 * wave + 4 * i + bq;
+        const int off = c * BL_BK + 16 * ((bk2 >> 4) ^ ((c >> 3) & 1)) + (bk2 & 15);
+        int32_t x0 = bl_residue(vb[2 * i], mp, top), x1 = bl_residue(vb[2 * i + 1], mp, top);
+#pragma unroll
+        for (int u = 0; u < L; ++u) {
+            *reinterpret_cast<uint16_t *>(&Bs[u][off]) = (uint16_t)(((uint32_t)x0 & 255u) | (((uint32_t)x1 & 255u) << 8));
+            x0 = (x0 + 128) >> 8;
+            x1 = (x1 + 128) >> 8;
+        }
+    }
+}
+
+// Epilogue of one 32 x 32 block held transposed (lane l: row i = l % 32 of C,
+// columns j0 + 8 (e / 4) + 4 (l / 32) + e % 4 of element e):
+// sum_s (acc_s mod p) (256^s mod p) mod p in FP64, stored as T.
+template <typename T, int NA>
+__device__ __forceinline__ void bl_store_block(const bl_v16i (&acc)[NA], const BJob &j, int64_t i, int64_t j0, int h)
+{
+    const ModP mp = j.mp;
+    T *C = (T *)j.C;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int64_t jj = j0 + 8 * (e >> 2) + 4 * h + (e & 3);
+        double S = 0.0;
+        if (j.lazy) {
+            // |S| < 2^51: the rounded quotient is within one of the exact
+            // one, so one fix-up gives S mod p
+#pragma unroll
+            for (int t = 0; t < NA; ++t) S = fma((double)acc[t][e], j.w[t], S);
+            S = fma(-rint(S * mp.pinv), mp.pd, S);
+            S = S < 0.0 ? S + mp.pd : S;
+        } else {
+#pragma unroll
+            for (int t = 0; t < NA; ++t) S = dmod(fma(dmod((double)acc[t][e], mp), j.w[t], S), mp);
+        }
+        // S is an exact integer in [0, p)
+        if (i < j.m && jj < j.n) C[i + jj * j.ldc] = (T)S;
+    }
+}
+
 template <typename T, int L>
 __global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict__ jobs,
                                                          const BItem *__restrict__ items, XcdRanges xr,
                                                          int *__restrict__ counter)
 {
-    typedef int v2i __attribute__((ext_vector_type(2)));
     typedef int v4i __attribute__((ext_vector_type(4)));
-    typedef int v16i __attribute__((ext_vector_type(16)));
+    typedef bl_v16i v16i;
     constexpr int TL = BL_TILE, BK = BL_BK, NA = 2 * L - 1;
     constexpr int32_t TOP = (int32_t)(127 * ((((int64_t)1) << (8 * L)) - 1) / 255);
     __shared__ __attribute__((aligned(16))) int8_t As[2][L][TL * BK];
@@ -181,47 +265,12 @@ __global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict_
                     }
                 }
             };
-            // residue of a stored element: a compare when it is already in
-            // [0, p) (the usual case), the FP64 reduction otherwise
-            auto res = [&](T v) -> int32_t {
-                const double x = (double)v;
-                uint32_t r = (uint32_t)x;
-                if (!(x >= 0.0 && x < mp.pd)) r = (uint32_t)dmod(x, mp);
-                return digit_rep(r, TOP, mp);
-            };
-            // Digits: the stored byte of digit u of x is the low byte of x_u,
-            // with x_0 = x and x_{u+1} = (x_u + 128) >> 8 (arithmetic), the
-            // signed base-256 digits of the limb form.
             auto lstore = [&](int buf) {
                 int32_t xa[8];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) xa[e] = res(va[e]);
-#pragma unroll
-                for (int u = 0; u < L; ++u) {
-                    v2i wa;
-#pragma unroll
-                    for (int d = 0; d < 2; ++d)
-                        wa[d] = (int)(((uint32_t)xa[4 * d] & 255u) | (((uint32_t)xa[4 * d + 1] & 255u) << 8) |
-                                      (((uint32_t)xa[4 * d + 2] & 255u) << 16) | ((uint32_t)xa[4 * d + 3] << 24));
-                    *reinterpret_cast<v2i *>(&As[buf][u][aslot]) = wa;
-                    if (u + 1 < L) {
-#pragma unroll
-                        for (int e = 0; e < 8; ++e) xa[e] = (xa[e] + 128) >> 8;
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int c = 16 * wave + 4 * i + bq;
-                    const int off = c * BK + 16 * ((bk2 >> 4) ^ ((c >> 3) & 1)) + (bk2 & 15);
-                    int32_t x0 = res(vb[2 * i]), x1 = res(vb[2 * i + 1]);
-#pragma unroll
-                    for (int u = 0; u < L; ++u) {
-                        *reinterpret_cast<uint16_t *>(&Bs[buf][u][off]) =
-                            (uint16_t)(((uint32_t)x0 & 255u) | (((uint32_t)x1 & 255u) << 8));
-                        x0 = (x0 + 128) >> 8;
-                        x1 = (x1 + 128) >> 8;
-                    }
-                }
+                for (int e = 0; e < 8; ++e) xa[e] = bl_residue(va[e], mp, TOP);
+                bl_store_a8<L>(xa, [&](int u) { return &As[buf][u][aslot]; });
+                bl_store_b<T, L>(vb, Bs[buf], wave, bq, bk2, mp, TOP);
             };
             v16i acc[NA];
 #pragma unroll
@@ -250,26 +299,7 @@ __global__ __launch_bounds__(256, 2) void k_batched_limb(const BJob *__restrict_
                 if (st + 1 < ns) lstore(buf ^ 1);
                 __syncthreads();
             }
-            const int64_t i = r0 + wr * 32 + fr;
-            T *C = (T *)j.C;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int64_t jj = c0 + wc * 32 + 8 * (e >> 2) + 4 * h + (e & 3);
-                double S = 0.0;
-                if (j.lazy) {
-                    // |S| < 2^51: the rounded quotient is within one of the
-                    // exact one, so one fix-up gives S mod p
-#pragma unroll
-                    for (int t = 0; t < NA; ++t) S = fma((double)acc[t][e], j.w[t], S);
-                    S = fma(-rint(S * mp.pinv), mp.pd, S);
-                    S = S < 0.0 ? S + mp.pd : S;
-                } else {
-#pragma unroll
-                    for (int t = 0; t < NA; ++t) S = dmod(fma(dmod((double)acc[t][e], mp), j.w[t], S), mp);
-                }
-                // S is an exact integer in [0, p)
-                if (i < j.m && jj < j.n) C[i + jj * j.ldc] = (T)S;
-            }
+            bl_store_block<T, NA>(acc, j, r0 + wr * 32 + fr, c0 + wc * 32, h);
         }
     }
 }
@@ -286,10 +316,9 @@ __global__ __launch_bounds__(256, L == 4 ? 1 : 2) void k_batched_limb_rp(const B
                                                             const BItem *__restrict__ items, XcdRanges xr,
                                                             int *__restrict__ counter)
 {
-    typedef int v2i __attribute__((ext_vector_type(2)));
     typedef int v4i __attribute__((ext_vector_type(4)));
-    typedef int v16i __attribute__((ext_vector_type(16)));
-    constexpr int TL = BL_TILE, BK = 32, NA = 2 * L - 1, AP = RP_KMAX + 16;
+    typedef bl_v16i v16i;
+    constexpr int TL = BL_TILE, BK = BL_BK, NA = 2 * L - 1, AP = RP_KMAX + 16;
     constexpr int32_t TOP = (int32_t)(127 * ((((int64_t)1) << (8 * L)) - 1) / 255);
     __shared__ __attribute__((aligned(16))) int8_t As[L][TL * AP];
     __shared__ __attribute__((aligned(16))) int8_t Bs[2][L][TL * BK];
@@ -316,14 +345,6 @@ __global__ __launch_bounds__(256, L == 4 ? 1 : 2) void k_batched_limb_rp(const B
             const T *A = (const T *)j.A, *B = (const T *)j.B;
             const ModP mp = j.mp;
             const int K = j.k, ns = (K + BK - 1) / BK;
-            auto res = [&](T v) -> int32_t {
-                const double x = (double)v;
-                uint32_t r = (uint32_t)x;
-                if (!(x >= 0.0 && x < mp.pd)) r = (uint32_t)dmod(x, mp);
-                return digit_rep(r, TOP, mp);
-            };
-            T *C = (T *)j.C;
-            const int64_t i = r0 + wr * 32 + fr;
             const int64_t arow = r0 + sr;
             const bool arow_ok = arow < j.m;
             for (int tj = 0; tj < j.tiles_n; ++tj) {
@@ -355,35 +376,10 @@ __global__ __launch_bounds__(256, L == 4 ? 1 : 2) void k_batched_limb_rp(const B
                     if (first) {
                         int32_t xa[8];
 #pragma unroll
-                        for (int e = 0; e < 8; ++e) xa[e] = res(va[e]);
-#pragma unroll
-                        for (int u = 0; u < L; ++u) {
-                            v2i wa;
-#pragma unroll
-                            for (int d = 0; d < 2; ++d)
-                                wa[d] = (int)(((uint32_t)xa[4 * d] & 255u) | (((uint32_t)xa[4 * d + 1] & 255u) << 8) |
-                                              (((uint32_t)xa[4 * d + 2] & 255u) << 16) |
-                                              ((uint32_t)xa[4 * d + 3] << 24));
-                            *reinterpret_cast<v2i *>(&As[u][sr * AP + st * BK + 8 * sc]) = wa;
-                            if (u + 1 < L) {
-#pragma unroll
-                                for (int e = 0; e < 8; ++e) xa[e] = (xa[e] + 128) >> 8;
-                            }
-                        }
+                        for (int e = 0; e < 8; ++e) xa[e] = bl_residue(va[e], mp, TOP);
+                        bl_store_a8<L>(xa, [&](int u) { return &As[u][sr * AP + st * BK + 8 * sc]; });
                     }
-#pragma unroll
-                    for (int ii = 0; ii < 4; ++ii) {
-                        const int c = 16 * wave + 4 * ii + bq;
-                        const int off = c * BK + 16 * ((bk2 >> 4) ^ ((c >> 3) & 1)) + (bk2 & 15);
-                        int32_t x0 = res(vb[2 * ii]), x1 = res(vb[2 * ii + 1]);
-#pragma unroll
-                        for (int u = 0; u < L; ++u) {
-                            *reinterpret_cast<uint16_t *>(&Bs[buf][u][off]) =
-                                (uint16_t)(((uint32_t)x0 & 255u) | (((uint32_t)x1 & 255u) << 8));
-                            x0 = (x0 + 128) >> 8;
-                            x1 = (x1 + 128) >> 8;
-                        }
-                    }
+                    bl_store_b<T, L>(vb, Bs[buf], wave, bq, bk2, mp, TOP);
                 };
                 v16i acc[NA];
 #pragma unroll
@@ -410,21 +406,7 @@ __global__ __launch_bounds__(256, L == 4 ? 1 : 2) void k_batched_limb_rp(const B
                     if (st + 1 < ns) lstore(buf ^ 1, st + 1);
                     __syncthreads();
                 }
-#pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const int64_t jj = c0 + wc * 32 + 8 * (e >> 2) + 4 * h + (e & 3);
-                    double S = 0.0;
-                    if (j.lazy) {
-#pragma unroll
-                        for (int t = 0; t < NA; ++t) S = fma((double)acc[t][e], j.w[t], S);
-                        S = fma(-rint(S * mp.pinv), mp.pd, S);
-                        S = S < 0.0 ? S + mp.pd : S;
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < NA; ++t) S = dmod(fma(dmod((double)acc[t][e], mp), j.w[t], S), mp);
-                    }
-                    if (i < j.m && jj < j.n) C[i + jj * j.ldc] = (T)S;
-                }
+                bl_store_block<T, NA>(acc, j, r0 + wr * 32 + fr, c0 + wc * 32, h);
             }
         }
     }
