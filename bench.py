@@ -317,14 +317,23 @@ def main():
         B7 = g.rand(np.float64, n, n, 127, seed=1, offset=n * n)
         C7a = g.zeros(np.float64, n, n, 127)
         C7b = g.zeros(np.float64, n, n, 127)
+        C7c = g.zeros(np.float64, n, n, 127)
         p127_limb_ms, _, _, _ = time_gemm(g, ctx, A7, B7, C7a, g.FORM_LIMB_I8, a.steps, a.warmup, "gemm_limb")
+        # one int8 product modulo 127 itself (multi-modular form, one plane)
+        p127_rns_ms, _, p127_rns_kms, _ = time_gemm(g, ctx, A7, B7, C7c, g.FORM_RNS_I8, a.steps, a.warmup,
+                                                    "gemm_rns")
         p127_plain_ms, _, _, _ = time_gemm(g, ctx, A7, B7, C7b, g.FORM_PLAIN_F64, max(2, a.steps // 2), 1,
                                            "gemm_f64")
+        c7b = g.Array(C7b)
         extra.update(p127_limb_gops=round(ops / (p127_limb_ms * 1e-3) / 1e9, 2),
+                     p127_rns_gops=round(ops / (p127_rns_ms * 1e-3) / 1e9, 2),
+                     p127_rns_core_ms=round(p127_rns_kms, 4) if p127_rns_kms else None,
                      p127_plain_gops=round(ops / (p127_plain_ms * 1e-3) / 1e9, 2),
                      p127_limb_vs_plain=round(p127_plain_ms / p127_limb_ms, 3),
-                     p127_max_abs_delta_limb_vs_plain=float(np.abs(g.Array(C7a) - g.Array(C7b)).max()))
-        del A7, B7, C7a, C7b
+                     p127_rns_vs_plain=round(p127_plain_ms / p127_rns_ms, 3),
+                     p127_max_abs_delta_limb_vs_plain=float(np.abs(g.Array(C7a) - c7b).max()),
+                     p127_max_abs_delta_rns_vs_plain=float(np.abs(g.Array(C7c) - c7b).max()))
+        del A7, B7, C7a, C7b, C7c, c7b
 
         # 1024^2 plain GEMM, p = 65521, seed 0
         A1 = g.rand(np.float64, 1024, 1024, 65521, seed=0)

@@ -368,6 +368,9 @@ size_t gemm_limb_scratch_bytes(const modp_ctx *ctx, const GemmArgs &g);
 // exact k-deep product mod p (0: none suffice), and the product with a
 // tile x tile output tile
 int rns_count(int64_t p, int64_t k);
+// residue planes of the RNS form: 1 for 2 <= p <= 255 (one int8 product
+// modulo p itself), else rns_count
+int rns_planes(int64_t p, int64_t k);
 int gemm_rns_run(modp_ctx *ctx, const GemmArgs &g, int tile);
 constexpr int64_t RNS_KCHUNK = 130944;
 int gemm_limb_i8_in(modp_ctx *ctx, const GemmArgs &g, const LimbScratch &ws);

@@ -375,7 +375,7 @@ bool gemm_config_valid(const GemmArgs &g, const GemmConfig &cfg)
     if (cfg.form == MODP_FORM_PLAIN_F64) return (sq64 || sq128) && f64_reduce_interval(g.mp.p) >= 1;
     if (cfg.form == MODP_FORM_RNS_I8) {
         const bool sq256 = cfg.tile_m == 256 && cfg.tile_n == 256;
-        return (sq128 || sq256) && cfg.ksplit == 1 && rns_count(g.mp.p, g.k < RNS_KCHUNK ? g.k : RNS_KCHUNK) > 0;
+        return (sq128 || sq256) && cfg.ksplit == 1 && rns_planes(g.mp.p, g.k < RNS_KCHUNK ? g.k : RNS_KCHUNK) > 0;
     }
     // 128x128 limb tiles hold 2L-1 accumulators of a 64x64 block per wave:
     // register room for L <= 2 only
@@ -387,12 +387,14 @@ bool gemm_config_valid(const GemmArgs &g, const GemmConfig &cfg)
 }
 
 // the RNS form's r moduli against the limb form's L^2 digit products: it
-// pays when it issues at most three quarters of the limb form's MFMA work
+// pays when it issues at most three quarters of the limb form's MFMA work,
+// and with one plane (p <= 255) against one digit product, whose core runs
+// at a lower fraction of the MFMA rate
 static bool rns_pays(const GemmArgs &g)
 {
     const int L = limb_count(g.mp.p);
-    const int r = rns_count(g.mp.p, g.k < RNS_KCHUNK ? g.k : RNS_KCHUNK);
-    return r > 0 && 4 * r <= 3 * L * L;
+    const int r = rns_planes(g.mp.p, g.k < RNS_KCHUNK ? g.k : RNS_KCHUNK);
+    return r > 0 && (r == 1 || 4 * r <= 3 * L * L);
 }
 
 GemmConfig gemm_default_config(const modp_ctx *ctx, const GemmArgs &g, int form)
@@ -404,7 +406,7 @@ GemmConfig gemm_default_config(const modp_ctx *ctx, const GemmArgs &g, int form)
     if (c.form == MODP_FORM_RNS_I8) {
         // 256x256 tiles while the grid (tiles x moduli) still holds two
         // workgroups per CU
-        const int r = rns_count(g.mp.p, g.k < RNS_KCHUNK ? g.k : RNS_KCHUNK);
+        const int r = rns_planes(g.mp.p, g.k < RNS_KCHUNK ? g.k : RNS_KCHUNK);
         c.tile_m = c.tile_n = tiles_of(g, 256, 256) * r >= 2 * ctx->num_cus ? 256 : 128;
         return c;
     }

@@ -38,6 +38,7 @@ constexpr int QLIST[NMOD_MAX] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 22
 // per-launch constants of the r moduli (kernel argument)
 struct Moduli {
     int r;
+    int single;               // one plane with q_0 = p (2 <= p <= 255): C mod p directly
     int q[NMOD_MAX];
     int y[NMOD_MAX];          // (M/q_j)^{-1} mod q_j
     double w[NMOD_MAX];       // (M/q_j) mod p
@@ -82,6 +83,32 @@ template <int Q, bool mulY> __device__ __forceinline__ v4i pack16q(const float *
     return w;
 }
 
+// pack16q for a modulus q known at run time (the single-plane form, q = p
+// <= 255).  There |x| <= 127, so |t| < 4096 + 255 and t (1/q) is within
+// 2^-20 of t / q, far closer than a non-tie quotient is to a half-integer
+// (1 / (2q) for odd q, 1 / q for even q): r is the balanced residue, or +-q/2
+// at an exact tie of an even q; |r| <= 127 either way.
+template <bool mulY> __device__ __forceinline__ v4i pack16r(const float *xh, const float *xl, int q, int y)
+{
+    const float qf = (float)q, iq = 1.0f / qf;
+    const float C = mulY ? (float)((4096 * y) % q) : (float)(4096 % q);
+    const float yf = (float)y;
+    v4i w;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        uint32_t u = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float lo = mulY ? xl[4 * d + e] * yf : xl[4 * d + e];
+            const float t = fmaf(xh[4 * d + e], C, lo);
+            const int v = (int)fmaf(-rintf(t * iq), qf, t);
+            u |= ((uint32_t)v & 255u) << (8 * e);
+        }
+        w[d] = (int)u;
+    }
+    return w;
+}
+
 // x -> (xh, xl) as floats: x = xh 4096 + xl with 0 <= xl < 4096
 __device__ __forceinline__ void split12(int x, float &xh, float &xl)
 {
@@ -92,6 +119,7 @@ __device__ __forceinline__ void split12(int x, float &xh, float &xl)
 // the same for modulus q_j (a wave-uniform j: one branch per plane)
 template <bool mulY> __device__ __forceinline__ v4i pack16(const float *xh, const float *xl, int j, const Moduli &md)
 {
+    if (md.single) return pack16r<mulY>(xh, xl, md.q[0], md.y[0]);
     switch (j) {
     case 0: return pack16q<QLIST[0], mulY>(xh, xl, md.y[0]);
     case 1: return pack16q<QLIST[1], mulY>(xh, xl, md.y[1]);

@@ -40,6 +40,28 @@ int rns_count(int64_t p, int64_t k)
     return 0;
 }
 
+// 2 <= p <= 255: one int8 product with the modulus p itself.  Balanced
+// residues (|x| <= 127) keep a K chunk's int32 sums exact, and the plane
+// holds C mod p, so no Chinese remaindering is needed.
+bool rns_single(int64_t p) { return p >= 2 && p <= 255; }
+
+int rns_planes(int64_t p, int64_t k) { return rns_single(p) ? 1 : rns_count(p, k); }
+
+// the single plane as a one-modulus system with M = q_0 = p: y_0 = 1,
+// w_0 = 1 and M mod p = 0, so the recombination returns the plane's residue
+static Moduli single_modulus(int64_t p)
+{
+    Moduli md{};
+    md.r = 1;
+    md.single = 1;
+    md.q[0] = (int)p;
+    md.y[0] = 1;
+    md.w[0] = 1.0;
+    md.invq[0] = 1.0 / (double)p;
+    md.wm = 0.0;
+    return md;
+}
+
 static Moduli make_moduli(int64_t p, int r)
 {
     Moduli md{};
@@ -133,10 +155,10 @@ int gemm_rns_run(modp_ctx *ctx, const GemmArgs &g, int tile)
     do {
         int64_t kc = g.k - kdone;
         if (kc > kc_max) kc = kc_max;
-        const int r = rns_count(g.mp.p, kc > 0 ? kc : 1);
+        const int r = rns_planes(g.mp.p, kc > 0 ? kc : 1);
         if (r == 0) return set_error(ctx, MODP_ERR_MODULUS_RANGE, "modulus %lld too large for the RNS form",
                                      (long long)g.mp.p);
-        const Moduli md = make_moduli(g.mp.p, r);
+        const Moduli md = rns_single(g.mp.p) ? single_modulus(g.mp.p) : make_moduli(g.mp.p, r);
         const int64_t Kp = round_up(kc > 0 ? kc : 1, 128);
         int st = MODP_OK;
         int8_t *Yp = (int8_t *)ws_get(ctx, WS_ALIMB, (size_t)r * Mp * Kp, &st);

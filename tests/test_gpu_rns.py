@@ -1,7 +1,8 @@
 """The multi-modular (RNS) int8 form of the mod-p GEMM against the oracle:
 shapes, moduli on both sides of each modulus-count step, the extreme
-residues that reach the reconstruction bound, K chunks, element types and
-the elimination's subtracting epilogue (through the autotuner's configs)."""
+residues that reach the reconstruction bound, K chunks, element types, the
+single-plane form of moduli up to 255 and the elimination's subtracting
+epilogue (through the autotuner's configs)."""
 import numpy as np
 import pytest
 
@@ -96,6 +97,48 @@ def test_rns_explicit_configs(gffm, tile):
     assert ctx.lib.modp_gemm_ex(ctx.h, ctypes.byref(bad), 0, m, n, k, A.ptr, A.ld, B.ptr, B.ld, C.ptr, C.ld, p) != 0
 
 
+@pytest.mark.parametrize("p", [2, 3, 4, 9, 127, 128, 200, 251, 254, 255])
+@pytest.mark.parametrize("fill", ["half", "half_plus", "pm1", "mixed"])
+def test_rns_single_plane_extremes(gffm, p, fill):
+    # 2 <= p <= 255: one int8 product modulo p itself (balanced residues up to
+    # 127 in magnitude, ties of even moduli at +-p/2), composite moduli too
+    g = gffm
+    h = p // 2
+    m, n, k = 66, 70, 3000
+    val = {"half": h, "half_plus": (h + 1) % p, "pm1": p - 1}.get(fill)
+    if fill == "mixed":
+        rng = np.random.default_rng(p + 7)
+        Ah = rng.choice([h, (h + 1) % p, 0, p - 1], size=(m, k))
+        Bh = rng.choice([h, (h + 1) % p, 1, p - 1], size=(k, n))
+    else:
+        Ah = np.full((m, k), val, dtype=np.int64)
+        Bh = np.full((k, n), val, dtype=np.int64)
+    assert np.array_equal(rns_product(g, Ah, Bh, p), O.gemm(Ah, Bh, p))
+
+
+@pytest.mark.parametrize("p", [251, 255])
+def test_rns_single_plane_k_chunks(gffm, p):
+    # two int32-exact chunks (130944 + rest) at the largest balanced
+    # magnitude, 127^2 per product in the first chunk's int32 sums
+    g = gffm
+    m, n, k = 5, 6, 131000
+    Ah = O.draw_matrix(m, k, p, seed=26)
+    Bh = O.draw_matrix(k, n, p, seed=26, offset=m * k)
+    Ah[0, :] = p // 2
+    Bh[:, 0] = p // 2
+    Ah[1, :] = (p + 1) // 2
+    Bh[:, 1] = (p + 1) // 2
+    assert np.array_equal(rns_product(g, Ah, Bh, p), O.gemm(Ah, Bh, p))
+
+
+def test_rns_single_plane_float32(gffm):
+    g = gffm
+    p = 251
+    Ah = O.draw_matrix(300, 5000, p, seed=27)
+    Bh = O.draw_matrix(5000, 200, p, seed=27, offset=300 * 5000)
+    assert np.array_equal(rns_product(g, Ah, Bh, p, np.float32), O.gemm(Ah, Bh, p))
+
+
 def test_auto_picks_rns_for_large_moduli(gffm):
     g = gffm
     ctx = g.context()
@@ -103,3 +146,6 @@ def test_auto_picks_rns_for_large_moduli(gffm):
     assert cfg["form"] == RNS and cfg["tile_m"] == 256
     cfg, _ = ctx.gemm_config(8192, 8192, 8192, 65521)
     assert cfg["form"] == 2
+    # moduli up to 255: one plane, chosen over the one-digit limb form
+    cfg, _ = ctx.gemm_config(8192, 8192, 8192, 127)
+    assert cfg["form"] == RNS and cfg["tile_m"] == 256
