@@ -511,4 +511,32 @@ __global__ __launch_bounds__(256) void k_combine(int64_t m, int64_t n, const uin
     }
 }
 
+// One plane (q_0 = p <= 255): E holds C mod p itself.  Each wave takes 1024
+// rows of one column, lanes along the rows (64 consecutive bytes per load,
+// 512 bytes per 8-byte store).
+template <typename TC, int EPI>
+__global__ __launch_bounds__(256) void k_combine1(int64_t m, int64_t n, const uint8_t *__restrict__ E, int64_t Mp,
+                                                 TC *__restrict__ C, int64_t ldc, ModP mp,
+                                                 const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int64_t chunks = (m + 1023) / 1024;
+    const int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t i = wv / chunks;
+    if (i >= n) return;
+    const int64_t base = (wv % chunks) * 1024 + (threadIdx.x & 63);
+    const uint8_t *src = E + i * Mp;
+    TC *dst = C + i * ldc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int64_t row = base + 64 * e;
+        if (row < m) {
+            uint32_t r = src[row];
+            if (EPI == EPI_SUB) r = submod(to_res<TC>(dst[row], mp), r, mp);
+            else if (EPI == EPI_ADD) r = addmod(to_res<TC>(dst[row], mp), r, mp);
+            dst[row] = from_res<TC>(r);
+        }
+    }
+}
+
 }  // namespace rns

@@ -117,8 +117,19 @@ template <typename TC>
 int combine_t(modp_ctx *ctx, const GemmArgs &g, int epi, const uint8_t *E, int64_t Mp, int64_t plane,
               const Moduli &md)
 {
-    const unsigned grid = (unsigned)((((g.m + 3) / 4) * g.n + 255) / 256);
     TC *C = (TC *)g.C;
+    if (md.single) {
+        // one wave per 1024 rows of a column, four per workgroup
+        const unsigned g1 = (unsigned)((((g.m + 1023) / 1024) * g.n + 3) / 4);
+        if (epi == EPI_STORE)
+            rns::k_combine1<TC, EPI_STORE><<<g1, 256, 0, ctx->stream>>>(g.m, g.n, E, Mp, C, g.ldc, g.mp, g.skip);
+        else if (epi == EPI_SUB)
+            rns::k_combine1<TC, EPI_SUB><<<g1, 256, 0, ctx->stream>>>(g.m, g.n, E, Mp, C, g.ldc, g.mp, g.skip);
+        else
+            rns::k_combine1<TC, EPI_ADD><<<g1, 256, 0, ctx->stream>>>(g.m, g.n, E, Mp, C, g.ldc, g.mp, g.skip);
+        return hip_check(ctx, hipGetLastError(), "k_combine1 launch");
+    }
+    const unsigned grid = (unsigned)((((g.m + 3) / 4) * g.n + 255) / 256);
     if (epi == EPI_STORE)
         rns::k_combine<TC, EPI_STORE><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, E, Mp, plane, md, C, g.ldc, g.mp, g.skip);
     else if (epi == EPI_SUB)
