@@ -474,7 +474,7 @@ __global__ __launch_bounds__(256) void k_split_cols(int64_t k, int64_t n, const 
 template <typename TC, int EPI>
 __global__ __launch_bounds__(256) void k_combine(int64_t m, int64_t n, const uint8_t *__restrict__ E, int64_t Mp,
                                                 int64_t plane, Moduli md, TC *__restrict__ C, int64_t ldc,
-                                                ModP mp, const int *__restrict__ skip)
+                                                ModP mp, const int *__restrict__ skip, bool vec_ok)
 {
     if (skip && *skip) return;
     // four consecutive rows of one column per thread: one 4-byte load per
@@ -498,17 +498,27 @@ __global__ __launch_bounds__(256) void k_combine(int64_t m, int64_t n, const uin
             }
         }
     }
+    TC *dst = C + j0 + i * ldc;
+    uint32_t r[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        if (j0 + e < m) {
-            const double t = rint(T[e]);
-            uint32_t r = (uint32_t)dmod(fma(-t, md.wm, S[e]), mp);
-            TC *dst = C + j0 + e + i * ldc;
-            if (EPI == EPI_SUB) r = submod(to_res<TC>(*dst, mp), r, mp);
-            else if (EPI == EPI_ADD) r = addmod(to_res<TC>(*dst, mp), r, mp);
-            *dst = from_res<TC>(r);
+        r[e] = (uint32_t)dmod(fma(-rint(T[e]), md.wm, S[e]), mp);
+        if (EPI != EPI_STORE && j0 + e < m) {
+            const uint32_t c = to_res<TC>(dst[e], mp);
+            r[e] = EPI == EPI_SUB ? submod(c, r[e], mp) : addmod(c, r[e], mp);
         }
     }
+    // four whole rows at a 16-byte aligned address (float64, ldc even and C
+    // aligned, which the launch checks): two 16-byte stores
+    if (sizeof(TC) == 8 && vec_ok && j0 + 3 < m) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        *reinterpret_cast<d2 *>(dst) = d2{(double)r[0], (double)r[1]};
+        *reinterpret_cast<d2 *>(dst + 2) = d2{(double)r[2], (double)r[3]};
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+        if (j0 + e < m) dst[e] = from_res<TC>(r[e]);
 }
 
 // One plane (q_0 = p <= 255): E holds C mod p itself.  Each wave takes 1024

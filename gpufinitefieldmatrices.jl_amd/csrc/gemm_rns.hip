@@ -130,12 +130,17 @@ int combine_t(modp_ctx *ctx, const GemmArgs &g, int epi, const uint8_t *E, int64
         return hip_check(ctx, hipGetLastError(), "k_combine1 launch");
     }
     const unsigned grid = (unsigned)((((g.m + 3) / 4) * g.n + 255) / 256);
+    // 16-byte stores of four rows need every column start 16-byte aligned
+    const bool vec = sizeof(TC) == 8 && (g.ldc & 1) == 0 && ((uintptr_t)C & 15) == 0;
     if (epi == EPI_STORE)
-        rns::k_combine<TC, EPI_STORE><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, E, Mp, plane, md, C, g.ldc, g.mp, g.skip);
+        rns::k_combine<TC, EPI_STORE><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, E, Mp, plane, md, C, g.ldc, g.mp, g.skip,
+                                                                      vec);
     else if (epi == EPI_SUB)
-        rns::k_combine<TC, EPI_SUB><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, E, Mp, plane, md, C, g.ldc, g.mp, g.skip);
+        rns::k_combine<TC, EPI_SUB><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, E, Mp, plane, md, C, g.ldc, g.mp, g.skip,
+                                                                    vec);
     else
-        rns::k_combine<TC, EPI_ADD><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, E, Mp, plane, md, C, g.ldc, g.mp, g.skip);
+        rns::k_combine<TC, EPI_ADD><<<grid, 256, 0, ctx->stream>>>(g.m, g.n, E, Mp, plane, md, C, g.ldc, g.mp, g.skip,
+                                                                    vec);
     return hip_check(ctx, hipGetLastError(), "k_combine launch");
 }
 
