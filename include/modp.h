@@ -54,7 +54,8 @@ typedef enum {
     MODP_FORM_RNS_I8 = 3     /* multi-modular limbs: residues modulo up to 12 pairwise
                                 coprime moduli <= 256 in int8, one int32-accumulating
                                 MFMA product per modulus, Chinese-remainder
-                                recombination in FP64 (GEMM only) */
+                                recombination in FP64 (GEMM only); for
+                                2 <= p <= 255 one product modulo p itself */
 } modp_form;
 
 /* autotuner modes (modp_autotune_mode) */
