@@ -177,10 +177,35 @@ def test_unreduced_inputs_are_reduced(gffm):
     Bh = np.arange(100).reshape(10, 10) * 1000
     A = g.CuModMatrix(Ah, p, mod=False, elem_type=np.float64)
     B = g.CuModMatrix(Bh, p, mod=False, elem_type=np.float64)
-    for form in (1, 2):
+    for form in (1, 2, 3):
         C = g.zeros(np.float64, 10, 10, p)
         g.mul_(C, A, B, form=form)
         assert np.array_equal(g.Array(C).astype(np.int64), O.matmul_bigint(np.mod(Ah, p), np.mod(Bh, p), p))
+
+
+@pytest.mark.parametrize("et", [np.float64, np.float32])
+@pytest.mark.parametrize("p", [101, 251, 65521, 2 ** 26 - 5])
+def test_unreduced_inputs_every_form(gffm, p, et):
+    # waves that mix reduced elements with negative ones, ones >= p and large
+    # magnitudes (exact in the storage type), in every form
+    if et == np.float32 and p > 2897:
+        pytest.skip("float32 storage: the reference's stripe rule admits p <= 2897")
+    g = gffm
+    rng = np.random.default_rng(p % 997)
+    m, n, k = 130, 70, 300
+    big = 2 ** 40 if et == np.float64 else 2 ** 23
+    Ah = rng.integers(-big, big, size=(m, k))
+    Bh = rng.integers(-3 * p, 3 * p, size=(k, n))
+    Ah[::2, :] = np.mod(Ah[::2, :], p)
+    Bh[:, ::3] = np.mod(Bh[:, ::3], p)
+    Bh[5, :] = p
+    A = g.CuModMatrix(Ah, p, mod=False, elem_type=et)
+    B = g.CuModMatrix(Bh, p, mod=False, elem_type=et)
+    ref = O.gemm(np.mod(Ah, p), np.mod(Bh, p), p)
+    for form in (1, 2, 3):
+        C = g.zeros(et, m, n, p)
+        g.mul_(C, A, B, form=form)
+        assert np.array_equal(g.Array(C).astype(np.int64), ref), f"form {form}"
 
 
 def test_device_rand_matches_oracle_generator(gffm):
