@@ -153,6 +153,25 @@ def test_inverse_4096_freivalds(gffm):
         assert np.array_equal(O._mod_matvec(Ah, y, p), x)
 
 
+def test_inverse_4096_small_prime_freivalds(gffm):
+    # p <= 255 at full size: the 2048-wide triangular-inverse levels run
+    # their products (store, then subtract) as one int8 product modulo p
+    g = gffm
+    p, n = 251, 4096
+    for seed in (2, 3, 4):
+        A = g.rand(np.float64, n, n, p, seed=seed)
+        if g.is_invertible(A):
+            break
+    X = g.inverse(A)
+    Ah = g.Array(A).astype(np.int64)
+    Xh = g.Array(X).astype(np.int64)
+    rng = np.random.default_rng(1)
+    for _ in range(2):
+        x = rng.integers(0, p, size=n)
+        y = O._mod_matvec(Xh, x, p)
+        assert np.array_equal(O._mod_matvec(Ah, y, p), x)
+
+
 @pytest.mark.parametrize("p1,p2", [(2, 3), (65521, 65519), (67108859, 67108837), (8388593, 7)])
 def test_crt_two_moduli(gffm, p1, p2):
     # per-prime products combined: CRT(A*B mod p1, A*B mod p2) = A*B mod p1*p2

@@ -146,6 +146,9 @@ def test_auto_picks_rns_for_large_moduli(gffm):
     assert cfg["form"] == RNS and cfg["tile_m"] == 256
     cfg, _ = ctx.gemm_config(8192, 8192, 8192, 65521)
     assert cfg["form"] == 2
-    # moduli up to 255: one plane, chosen over the one-digit limb form
+    # moduli up to 255: one plane, chosen over the one-digit limb form while
+    # its grid fills the chip
     cfg, _ = ctx.gemm_config(8192, 8192, 8192, 127)
     assert cfg["form"] == RNS and cfg["tile_m"] == 256
+    cfg, _ = ctx.gemm_config(1024, 1024, 1024, 127)
+    assert cfg["form"] == 2
