@@ -401,10 +401,11 @@ GemmConfig gemm_default_config(const modp_ctx *ctx, const GemmArgs &g, int form)
 {
     GemmConfig c;
     c.form = form == MODP_FORM_AUTO ? choose_form(g.m, g.n, g.k, g.mp.p) : form;
-    // (one plane only while its grid of 128 x 128 tiles covers every CU:
-    // below that the one-digit form's smaller tiles fill the chip)
+    // (one plane only with at least two 256 x 256 output tiles per CU:
+    // measured at p = 127, the one-digit form is faster at 4096^2 -- 646
+    // against 592 TOPS -- and the one plane at 8192^2, 1060 against 967)
     if (form == MODP_FORM_AUTO && c.form == MODP_FORM_LIMB_I8 && rns_pays(g) &&
-        (rns_planes(g.mp.p, g.k < RNS_KCHUNK ? g.k : RNS_KCHUNK) > 1 || tiles_of(g, 128, 128) >= ctx->num_cus))
+        (rns_planes(g.mp.p, g.k < RNS_KCHUNK ? g.k : RNS_KCHUNK) > 1 || tiles_of(g, 256, 256) >= 2 * ctx->num_cus))
         c.form = MODP_FORM_RNS_I8;
     c.ksplit = 1;
     if (c.form == MODP_FORM_RNS_I8) {

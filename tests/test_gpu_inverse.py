@@ -154,8 +154,8 @@ def test_inverse_4096_freivalds(gffm):
 
 
 def test_inverse_4096_small_prime_freivalds(gffm):
-    # p <= 255 at full size: the 2048-wide triangular-inverse levels run
-    # their products (store, then subtract) as one int8 product modulo p
+    # p <= 255 at full size: PLUQ with 8-bit residues and the triangular
+    # inverses' products in the int8 digit form
     g = gffm
     p, n = 251, 4096
     for seed in (2, 3, 4):

@@ -150,5 +150,6 @@ def test_auto_picks_rns_for_large_moduli(gffm):
     # its grid fills the chip
     cfg, _ = ctx.gemm_config(8192, 8192, 8192, 127)
     assert cfg["form"] == RNS and cfg["tile_m"] == 256
-    cfg, _ = ctx.gemm_config(1024, 1024, 1024, 127)
-    assert cfg["form"] == 2
+    for n in (1024, 4096):
+        cfg, _ = ctx.gemm_config(n, n, n, 127)
+        assert cfg["form"] == 2
