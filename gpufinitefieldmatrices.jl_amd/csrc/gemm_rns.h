@@ -390,9 +390,27 @@ __global__ __launch_bounds__(256) void k_split_rows(int64_t m, int64_t k, const 
     __shared__ int32_t tile[TR * LDT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t r0 = (int64_t)blockIdx.y * TR, k0 = (int64_t)blockIdx.x * TK;
-    {
-        const int rr = 2 * (lane & 31);
-        const int64_t gr = r0 + rr;
+    const int rr = 2 * (lane & 31);
+    const int64_t gr = r0 + rr;
+    // a tile wholly inside A with every row pair aligned (the same for the
+    // whole workgroup): all sixteen pair loads are issued before the first
+    // is used, instead of one memory round trip per load
+    const bool interior = r0 + TR <= m && k0 + TK <= k && (lda & 1) == 0 &&
+                          (((uintptr_t)A + (size_t)r0 * sizeof(TA)) & (2 * sizeof(TA) - 1)) == 0;
+    if (interior) {
+        TA v[TK / 8][2];
+#pragma unroll
+        for (int it = 0; it < TK / 8; ++it) {
+            const int kk = 8 * it + 2 * wave + (lane >> 5);
+            __builtin_memcpy(v[it], A + gr + (k0 + kk) * lda, sizeof(v[it]));
+        }
+#pragma unroll
+        for (int it = 0; it < TK / 8; ++it) {
+            const int kk = 8 * it + 2 * wave + (lane >> 5);
+            tile[rr * LDT + kk] = balanced<TA>(v[it][0], mp);
+            tile[(rr + 1) * LDT + kk] = balanced<TA>(v[it][1], mp);
+        }
+    } else {
         const bool pair = gr + 1 < m && (((uintptr_t)A + (size_t)gr * sizeof(TA)) & (2 * sizeof(TA) - 1)) == 0 &&
                           (lda & 1) == 0;
 #pragma unroll 4
@@ -486,10 +504,16 @@ __global__ __launch_bounds__(256) void k_combine(int64_t m, int64_t n, const uin
     const int64_t j0 = 4 * (idx % mq), i = idx / mq;
     const uint8_t *src = E + i * Mp + j0;
     double S[4] = {0.0, 0.0, 0.0, 0.0}, T[4] = {0.0, 0.0, 0.0, 0.0};
+    // every plane's word loaded before the first is used (planes past the
+    // r-th re-read the last one, so no load waits behind a branch)
+    uint32_t w[NMOD_MAX];
+#pragma unroll
+    for (int jj = 0; jj < NMOD_MAX; ++jj)
+        w[jj] = *reinterpret_cast<const uint32_t *>(src + (size_t)(jj < md.r ? jj : md.r - 1) * plane);
 #pragma unroll
     for (int jj = 0; jj < NMOD_MAX; ++jj) {
         if (jj < md.r) {
-            const uint32_t w4 = *reinterpret_cast<const uint32_t *>(src + (size_t)jj * plane);
+            const uint32_t w4 = w[jj];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const double v = (double)((w4 >> (8 * e)) & 255u);
