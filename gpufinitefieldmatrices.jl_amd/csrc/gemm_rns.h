@@ -561,6 +561,15 @@ __global__ __launch_bounds__(256) void k_combine1(int64_t m, int64_t n, const ui
     const int64_t base = (wv % chunks) * 1024 + (threadIdx.x & 63);
     const uint8_t *src = E + i * Mp;
     TC *dst = C + i * ldc;
+    if (EPI == EPI_STORE && base - (threadIdx.x & 63) + 1024 <= m) {
+        // all 1024 rows inside the column: the 16 loads go out together
+        uint32_t r[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) r[e] = src[base + 64 * e];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) dst[base + 64 * e] = from_res<TC>(r[e]);
+        return;
+    }
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
         const int64_t row = base + 64 * e;
