@@ -66,13 +66,24 @@ __global__ __launch_bounds__(256) void k_split_a(int64_t m, int64_t k, const TA 
     __shared__ int32_t tile[64][65];
     const int tid = threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 64;
+    if (r0 + 64 <= m && k0 + 64 <= k) {
+        // a tile wholly inside A (the same for the whole workgroup): the 16
+        // loads of a thread go out together
+        const int row = tid & 63;
+        TA x[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        int row = tid & 63, kk = (tid >> 6) + 4 * i;
-        int64_t gr = r0 + row, gk = k0 + kk;
-        int32_t v = 0;
-        if (gr < m && gk < k) v = digit_rep(to_res<TA>(A[gr + gk * lda], mp), top, mp);
-        tile[row][kk] = v;
+        for (int i = 0; i < 16; ++i) x[i] = A[r0 + row + (k0 + (tid >> 6) + 4 * i) * lda];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) tile[row][(tid >> 6) + 4 * i] = digit_rep(to_res<TA>(x[i], mp), top, mp);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            int row = tid & 63, kk = (tid >> 6) + 4 * i;
+            int64_t gr = r0 + row, gk = k0 + kk;
+            int32_t v = 0;
+            if (gr < m && gk < k) v = digit_rep(to_res<TA>(A[gr + gk * lda], mp), top, mp);
+            tile[row][kk] = v;
+        }
     }
     __syncthreads();
     const int row = tid >> 2, ch = tid & 3;
@@ -104,10 +115,24 @@ __global__ __launch_bounds__(256) void k_split_b(int64_t k, int64_t n, const TB 
     if (e >= Np * quads) return;
     const int64_t col = e / quads, k0 = (e % quads) * 4;
     uint32_t word[4] = {0u, 0u, 0u, 0u};
+    int32_t xs[4];
+    if (col < n && k0 + 4 <= k) {
+        // all four in range: the loads go out together
+        TB v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = B[k0 + i + col * ldb];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) xs[i] = digit_rep(to_res<TB>(v[i], mp), top, mp);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t gk = k0 + i;
+            xs[i] = (col < n && gk < k) ? digit_rep(to_res<TB>(B[gk + col * ldb], mp), top, mp) : 0;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int64_t gk = k0 + i;
-        int32_t x = (col < n && gk < k) ? digit_rep(to_res<TB>(B[gk + col * ldb], mp), top, mp) : 0;
+        int32_t x = xs[i];
 #pragma unroll
         for (int s = 0; s < 4; ++s) word[s] |= ((uint32_t)next_digit(x) & 255u) << (8 * i);
     }
