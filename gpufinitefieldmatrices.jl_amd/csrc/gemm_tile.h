@@ -73,34 +73,43 @@ __device__ __forceinline__ void gemm_f64_tile(int64_t m, int64_t n, int64_t k, c
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    double ra[APT], rb[BPT];
+    // a stage's raw elements, loaded at indices clamped into the operands
+    // (so no load waits behind a branch) with a bit per element that is in
+    // range; they are reduced and masked when stashed, one stage later
+    static_assert(APT <= 16 && BPT <= 16, "validity mask");
+    TA xa[APT];
+    TB xb[BPT];
+    uint32_t vmask = 0;
 
     auto load = [&](int64_t k0) {
+        vmask = 0;
 #pragma unroll
         for (int i = 0; i < APT; ++i) {
             int idx = tid + 256 * i;
             int row = idx % BM, kk = idx / BM;
             int64_t gr = r0 + row, gk = k0 + kk;
-            ra[i] = (gr < m && gk < k) ? bal_of<TA>(A[gr + gk * lda], mp) : 0.0;
+            xa[i] = A[min(gr, m - 1) + min(gk, k - 1) * lda];
+            vmask |= (gr < m && gk < k) ? 1u << i : 0u;
         }
 #pragma unroll
         for (int i = 0; i < BPT; ++i) {
             int idx = tid + 256 * i;
             int kk = idx % PBK, col = idx / PBK;
             int64_t gk = k0 + kk, gc = c0 + col;
-            rb[i] = (gk < k && gc < n) ? bal_of<TB>(B[gk + gc * ldb], mp) : 0.0;
+            xb[i] = B[min(gk, k - 1) + min(gc, n - 1) * ldb];
+            vmask |= (gk < k && gc < n) ? 1u << (16 + i) : 0u;
         }
     };
     auto stash = [&](int buf) {
 #pragma unroll
         for (int i = 0; i < APT; ++i) {
             int idx = tid + 256 * i;
-            sm.As[buf][idx / BM][idx % BM] = ra[i];
+            sm.As[buf][idx / BM][idx % BM] = ((vmask >> i) & 1u) ? bal_of<TA>(xa[i], mp) : 0.0;
         }
 #pragma unroll
         for (int i = 0; i < BPT; ++i) {
             int idx = tid + 256 * i;
-            sm.Bs[buf][idx % PBK][idx / PBK] = rb[i];
+            sm.Bs[buf][idx % PBK][idx / PBK] = ((vmask >> (16 + i)) & 1u) ? bal_of<TB>(xb[i], mp) : 0.0;
         }
     };
 
