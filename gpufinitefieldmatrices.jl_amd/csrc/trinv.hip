@@ -34,7 +34,16 @@ __global__ __launch_bounds__(TB) void k_trinv_diag(const int32_t *__restrict__ T
     const int c = threadIdx.x;
     const int64_t o = (int64_t)blockIdx.x * TB;
     const int nb = (int)(k - o < TB ? k - o : TB);
-    for (int j = 0; j < nb; ++j) S[c][j] = c < nb ? (uint32_t)T[(o + c) + (o + j) * ldt] : 0u;
+    // sixteen loads in flight at a time (indices clamped into the block, so
+    // no load waits behind a branch)
+    for (int j0 = 0; j0 < nb; j0 += 16) {
+        int32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = T[(o + min(c, nb - 1)) + (o + min(j0 + u, nb - 1)) * ldt];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (j0 + u < nb) S[c][j0 + u] = c < nb ? (uint32_t)v[u] : 0u;
+    }
     __syncthreads();
     if (c < nb) {
         const uint32_t d = S[c][c];
